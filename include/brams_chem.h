@@ -1,0 +1,143 @@
+/*
+ * brams_chem.h -- C-ABI of the MI355X CCATT gas-phase chemistry step.
+ *
+ * Drop-in boundary for BRAMS's RODAS3 chemistry solver (CHEMISTRY=4,
+ * mechanism RELACS_TUV).  The Fortran module mod_chem_spack_rodas3_dyndt in
+ * brams_amd/fortran/ keeps the reference interface of
+ *   chem_rodas3_dyndt   src/brams/ccatt/chem_spack_rodas3_dyndt.f90:41-131
+ * and calls brams_chem_rodas3_dyndt() below through ISO_C_BINDING.
+ *
+ * Conventions
+ *   - Grid arrays are Fortran column-major (k,i,j) with extents (m1,m2,m3);
+ *     jphoto is (nr_photo,m1,m2,m3).  Species follow the mechanism order
+ *     (src/brams/ccatt/RELACS_TUV/chem1_list.f90:63-109, O3=1 ... XO2=47).
+ *   - Transported species are in ppbm, non-transported ones in molec/cm^3
+ *     (chem_spack_rodas3_dyndt.f90:261-317).
+ *   - Indices passed in (ia..jz, transp_chem_index, ...) are 1-based like the
+ *     Fortran caller's.
+ *   - Every call is synchronous and returns a status: 0 ok, >0 count of
+ *     blocks with a numerical failure (zero pivot or non-finite state, the
+ *     reference would abort there: src/utils/lib/spSolve.c:158-164), <0 a
+ *     fatal error whose text brams_chem_last_error() returns.
+ *   - One call at a time per process (the reference keeps module state).
+ */
+#ifndef BRAMS_CHEM_H
+#define BRAMS_CHEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BRAMS_CHEM_NSPECIES 47
+#define BRAMS_CHEM_NREACT 128
+#define BRAMS_CHEM_NPHOTO 17
+
+/* split_method (chem_spack_rodas3_dyndt.f90:62, chemistry.f90 SPLIT_METHOD) */
+enum { BRAMS_SPLIT_SYMMETRIC = 0, BRAMS_SPLIT_SEQUENTIAL = 1, BRAMS_SPLIT_PARALLEL = 2 };
+/* PhotojMethod (chem1_list.f90:784); LUT is rejected (not the RELACS_TUV path) */
+enum { BRAMS_PHOTOJ_FAST_TUV = 0, BRAMS_PHOTOJ_FAST_JX = 1, BRAMS_PHOTOJ_LUT = 2 };
+
+/*
+ * Arguments of one chemistry step on one grid, host memory.  Field for field
+ * the arguments of chem_rodas3_dyndt (chem_spack_rodas3_dyndt.f90:41-131);
+ * arguments the RELACS_TUV path never reads (dn0, rcp, cosz, att, spack,
+ * spack_2d, extra3d, get_non_zeros, indexk/i/j, kij_index) are not carried.
+ */
+typedef struct brams_chem_args {
+    int32_t m1, m2, m3;          /* grid extents (k,i,j) */
+    int32_t ia, iz, ja, jz;      /* interior columns, 1-based inclusive */
+    float dtlt;                  /* dynamics step (s) */
+    int32_t n_dyn_chem;          /* N_DYN_CHEM: chemistry step = dtlt*N */
+    int32_t split_method;        /* BRAMS_SPLIT_* */
+    int32_t photoj_method;       /* BRAMS_PHOTOJ_* */
+    float cp, cpor, p00;         /* rconstants */
+    const float *pp, *pi0, *theta, *rv;   /* (m1,m2,m3) */
+    const double *jphoto;        /* (nr_photo,m1,m2,m3) */
+    int32_t nspecies, nr, nr_photo;
+    const float *weight;         /* (nspecies) molar weights */
+    int32_t n_transported, n_not_transported;
+    const int32_t *transp_chem_index;     /* (n_transported), 1-based species */
+    const int32_t *no_transp_chem_index;  /* (n_not_transported) */
+    float **sc_p;                /* nspecies pointers to (m1,m2,m3) state, in/out */
+    float **sc_t;                /* nspecies pointers to (m1*m2*m3) tendencies (PARALLEL, N=1), else NULL */
+    const float *const *sc_t_dyn;/* nspecies pointers (PARALLEL, N>1), else NULL */
+    int32_t nob;                 /* number of blocks (AllocIndex) */
+    int32_t maxblock_size;
+    const int32_t *block_end;    /* (nob) cells per block */
+    double *last_accepted_dt;    /* (nob) in/out, per-block step carried across calls */
+    const double *atol, *rtol;   /* (nspecies) */
+} brams_chem_args;
+
+/*
+ * The same step with every array already resident in device memory (HBM).
+ * Species state is one contiguous buffer [nspecies][m1*m2*m3] (species-major);
+ * sc_t / sc_t_dyn likewise.  last_accepted_dt and block_end live on device.
+ */
+typedef struct brams_chem_dev_args {
+    int32_t m1, m2, m3, ia, iz, ja, jz;
+    float dtlt;
+    int32_t n_dyn_chem, split_method, photoj_method;
+    float cp, cpor, p00;
+    const float *pp, *pi0, *theta, *rv;
+    const double *jphoto;
+    const float *weight;         /* device (nspecies) */
+    const int32_t *transported;  /* device (nspecies) 1 = transported */
+    float *sc_p;                 /* device [nspecies][m1*m2*m3] */
+    float *sc_t;                 /* device or NULL */
+    const float *sc_t_dyn;       /* device or NULL */
+    int32_t nob;
+    const int32_t *block_start;  /* device (nob+1) prefix sums of block sizes */
+    double *last_accepted_dt;    /* device (nob) */
+    const double *atol, *rtol;   /* device (nspecies) */
+    void *stream;                /* hipStream_t, NULL = default stream */
+} brams_chem_dev_args;
+
+/* Per-call counters (no reference counterpart; the reference prints nothing). */
+typedef struct brams_chem_stats {
+    int64_t cells;               /* cells integrated */
+    int64_t blocks;              /* blocks integrated */
+    int64_t substeps;            /* accepted sub-steps, summed over blocks */
+    int64_t attempts;            /* attempts (accepted + rejected), summed over blocks */
+    int64_t failed_blocks;       /* blocks with a zero pivot or non-finite state */
+    int32_t mask_nnz;            /* entries kept by the value-based sparsity mask */
+    float kernel_ms;             /* device time of the last step, all kernels */
+} brams_chem_stats;
+
+/* Select the HIP device and load the mechanism tables.  Idempotent. */
+int brams_chem_init(int device);
+
+/*
+ * Drop-in for chem_rodas3_dyndt (chem_spack_rodas3_dyndt.f90:41): integrate
+ * every interior cell of one grid over dtlt*N_DYN_CHEM with RODAS3, updating
+ * sc_p (or sc_t), last_accepted_dt in place.  Host arrays; the library copies
+ * them to and from the device.
+ */
+int brams_chem_rodas3_dyndt(const brams_chem_args *args);
+
+/* Device-resident variant of brams_chem_rodas3_dyndt (same semantics). */
+int brams_chem_rodas3_dyndt_device(const brams_chem_dev_args *args);
+
+/*
+ * Block partition of one grid, AllocIndex (chem_spack_utils.f90:39-183):
+ * interior cells in j->i->k order cut into floor(N/block_size) blocks, the
+ * remainder spread +1 over the first blocks.  Writes block_end[0..nob-1] when
+ * block_end is non-NULL and max_nob is large enough; returns nob (<0 on error).
+ */
+int brams_chem_alloc_index(int32_t block_size, int32_t m1, int32_t ia, int32_t iz,
+                           int32_t ja, int32_t jz, int32_t *block_end, int32_t max_nob);
+
+/* Counters of the last step. */
+int brams_chem_get_stats(brams_chem_stats *out);
+
+/* Text of the last error (thread-local, never NULL). */
+const char *brams_chem_last_error(void);
+
+/* Release device buffers. */
+void brams_chem_finalize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BRAMS_CHEM_H */

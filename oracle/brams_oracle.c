@@ -1,0 +1,692 @@
+/*
+ * ORACLE -- test infrastructure only (see brams_oracle.h).
+ *
+ * CPU restatement of src/brams/ccatt/chem_spack_rodas3_dyndt.f90 and the
+ * RELACS_TUV kernels, written to follow the reference's operation order so
+ * that the kernels agree bit for bit with the compiled reference.  The one
+ * deliberate difference is the LU pivot order: the reference re-derives a
+ * Markowitz order from cell 1 of block 1 on every block-1 attempt
+ * (rodas3:470-506, spFactor.c:173-300); this oracle uses one fixed
+ * diagonal Markowitz order of the structural pattern (the same rule the GPU
+ * code generator uses).  Pivot order changes round-off only; the test suite
+ * measures that against the reference's Sparse1.3 through
+ * ora_use_external_sparse().
+ */
+#define _GNU_SOURCE
+#include "brams_oracle.h"
+#include "relacs_tuv_tables.h"
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#define NS ORA_NS
+#define NR ORA_NR
+#define NP ORA_NPHOT
+
+/* ------------------------------------------------------------------ kinetic
+ * chem_spack_kinetic.f90:7-435.  One function per rate-law family; each keeps
+ * the reference's evaluation order. */
+
+/* Troe fall-off, e.g. kinetic.f90:126-133 (rk 29):
+ * Effko=k0 (T/300)^-n, Rapk=kinf (T/300)^-m,
+ * k = (Effko M/(1+Effko M/Rapk)) * Fc^(1/(1+log10(Effko M/Rapk)^2)) */
+static double ora_troe(double temp, double summ, const double *p)
+{
+    double effko = p[0] * pow(temp / 3.0e2, -(p[1]));
+    double rapk = p[2] * pow(temp / 3.0e2, -(p[3]));
+    double l = log10(effko * summ / rapk);
+    return (effko * summ / (1.0 + effko * summ / rapk)) * pow(p[4], 1.0 / (1.0 + l * l));
+}
+
+void ora_kinetic(double temp, double press, double vapp, const double *jphoto, double *rk)
+{
+    /* third body and water, kinetic.f90:67,74 */
+    const double summ = press * 7.243e16 / temp;
+    const double h2o = 29.0 * summ * vapp / (18.0 + 11.0 * vapp);
+    for (int r = 0; r < NR; r++) {
+        const double *p = ora_params[r];
+        double k = 0.0;
+        switch (ora_family[r]) {
+        case 0: /* never assigned (rk 97, kinetic.f90:356-358): stays 0 */
+            k = 0.0;
+            break;
+        case 1: /* photolysis, kinetic.f90:82-98 */
+            k = jphoto[(int)p[0] - 1];
+            break;
+        case 2: /* constant */
+            k = p[0];
+            break;
+        case 3: /* Arrhenius exp(a - b/T), e.g. kinetic.f90:101 */
+            k = exp(p[0] - p[1] / temp);
+            break;
+        case 4: /* exp(a + n ln T - b/T), kinetic.f90:232-237 */
+            k = exp(p[0] + p[1] * log(temp) - p[2] / temp);
+            break;
+        case 5: { /* sum of c exp(n ln T - b/T) terms plus a constant, kinetic.f90:238-251 */
+            int nt = (ora_nparams[r] - 1) / 3;
+            double s = 0.0;
+            for (int t = 0; t < nt; t++) {
+                double c = p[1 + 3 * t], n = p[2 + 3 * t], b = p[3 + 3 * t];
+                double x = (n != 0.0) ? n * log(temp) - b / temp : -(b / temp);
+                double term = c * exp(x);
+                s = (t == 0) ? term : s + term;
+            }
+            k = s + p[0];
+            break;
+        }
+        case 6: /* (M c)(T/300)^n, kinetic.f90:99 */
+            k = summ * p[0] * pow(temp / 3.0e2, p[1]);
+            break;
+        case 7: /* c1 e^(b1/T) + c2 M e^(b2/T), kinetic.f90:115-119 */
+            k = p[0] * exp(p[1] / temp) + p[2] * summ * exp(p[3] / temp);
+            break;
+        case 8: /* Troe */
+            k = ora_troe(temp, summ, p);
+            break;
+        case 9: { /* HNO3 + OH, kinetic.f90:181-184 */
+            double effko = p[0] * exp(p[1] / temp);
+            double rapk = p[2] * exp(p[3] / temp);
+            double fac = p[4] * exp(p[5] / temp) * summ;
+            k = effko + fac / (1.0 + fac / rapk);
+            break;
+        }
+        case 10: /* c (1 + d M), kinetic.f90:229 */
+            k = p[0] * (1.0 + p[1] * summ);
+            break;
+        }
+        switch (ora_post[r]) {
+        case 1: k = k * summ * ora_post_params[r][0]; break;          /* e.g. kinetic.f90:100 */
+        case 2: k = k * h2o; break;                                   /* kinetic.f90:109 */
+        case 3: k = ora_troe(temp, summ, ora_post_params[r]) * k; break; /* kinetic.f90:178 */
+        default: break;
+        }
+        rk[r] = k;
+    }
+}
+
+/* ------------------------------------------------------------------ fexchem
+ * rates: chem_spack_rates.f90:63-192, w_r = rk_r * y_a * y_b;
+ * fexchem: chem_spack_fexchem.f90:73-676, f_s = sum_r c_sr w_r in r order.
+ * Species whose row the reference never writes (13, 14, 16) stay 0. */
+void ora_fexchem(const double *rk, const double *y, double *f)
+{
+    double w[NR];
+    for (int r = 0; r < NR; r++) {
+        double v = rk[r];
+        if (ora_react[r][0] >= 0) v = v * y[ora_react[r][0]];
+        if (ora_react[r][1] >= 0) v = v * y[ora_react[r][1]];
+        w[r] = v;
+    }
+    for (int s = 0; s < NS; s++) {
+        double acc = 0.0;
+        for (int t = ora_prod_start[s]; t < ora_prod_start[s + 1]; t++) {
+            double term = ora_prod_c[t] * w[ora_prod_r[t]];
+            acc = (t == ora_prod_start[s]) ? term : acc + term;
+        }
+        f[s] = acc;
+    }
+}
+
+/* ------------------------------------------------------------------ Jacobian
+ * dratedc.f90:62-296 + jacdchemdc.f90:68-1191: J_ij = sum over reactions r
+ * (ascending) and over each reactant slot of r equal to j of
+ * c_ir * rk_r * y_other.  Entries the reference never assigns stay 0. */
+typedef struct { double c; int r; int other; } jterm;
+static int g_nje;                        /* number of Jacobian entries */
+static int g_je_i[NS * NS], g_je_j[NS * NS], g_je_start[NS * NS + 1];
+static jterm *g_jt;
+static unsigned char g_struct[NS][NS];   /* structural pattern incl. diagonal */
+
+/* LU symbolic data in internal (pivot) order */
+static int g_perm[NS];                   /* internal -> species */
+static unsigned char g_fill[NS][NS];     /* internal pattern after fill-in */
+static int g_nnz_lu;
+
+static pthread_once_t g_once = PTHREAD_ONCE_INIT;
+
+static void ora_init_tables(void)
+{
+    /* stoichiometric coefficient lookup */
+    static double S[NS][NR];
+    static unsigned char has[NS][NR];
+    memset(S, 0, sizeof S);
+    memset(has, 0, sizeof has);
+    for (int s = 0; s < NS; s++)
+        for (int t = ora_prod_start[s]; t < ora_prod_start[s + 1]; t++) {
+            S[s][ora_prod_r[t]] = ora_prod_c[t];
+            has[s][ora_prod_r[t]] = 1;
+        }
+    int cap = 4096, n = 0;
+    g_jt = (jterm *)malloc(sizeof(jterm) * cap);
+    g_nje = 0;
+    for (int i = 0; i < NS; i++)
+        for (int j = 0; j < NS; j++) {
+            int start = n;
+            for (int r = 0; r < NR; r++) {
+                if (!has[i][r]) continue;
+                for (int slot = 0; slot < 2; slot++) {
+                    if (ora_react[r][slot] != j) continue;
+                    if (n == cap) { cap *= 2; g_jt = (jterm *)realloc(g_jt, sizeof(jterm) * cap); }
+                    g_jt[n].c = S[i][r];
+                    g_jt[n].r = r;
+                    g_jt[n].other = ora_react[r][1 - slot];
+                    n++;
+                }
+            }
+            if (n > start) {
+                g_je_i[g_nje] = i;
+                g_je_j[g_nje] = j;
+                g_je_start[g_nje] = start;
+                g_nje++;
+            }
+        }
+    g_je_start[g_nje] = n;
+
+    memset(g_struct, 0, sizeof g_struct);
+    for (int e = 0; e < g_nje; e++) g_struct[g_je_i[e]][g_je_j[e]] = 1;
+    for (int i = 0; i < NS; i++) g_struct[i][i] = 1;
+
+    /* Markowitz order with diagonal pivots on the structural pattern:
+     * smallest (row count-1)*(col count-1) in the active submatrix, ties to
+     * the lowest species index; fill-in added as each pivot is eliminated. */
+    unsigned char P[NS][NS], active[NS];
+    memcpy(P, g_struct, sizeof P);
+    memset(active, 1, sizeof active);
+    for (int step = 0; step < NS; step++) {
+        int best = -1;
+        long bestprod = 0;
+        for (int c = 0; c < NS; c++) {
+            if (!active[c]) continue;
+            long rc = -1, cc = -1;
+            for (int j = 0; j < NS; j++) if (active[j] && P[c][j]) rc++;
+            for (int i = 0; i < NS; i++) if (active[i] && P[i][c]) cc++;
+            long prod = rc * cc;
+            if (best < 0 || prod < bestprod) { best = c; bestprod = prod; }
+        }
+        g_perm[step] = best;
+        active[best] = 0;
+        for (int i = 0; i < NS; i++) {
+            if (!active[i] || !P[i][best]) continue;
+            for (int j = 0; j < NS; j++)
+                if (active[j] && P[best][j]) P[i][j] = 1;
+        }
+    }
+    g_nnz_lu = 0;
+    for (int I = 0; I < NS; I++)
+        for (int J = 0; J < NS; J++) {
+            g_fill[I][J] = P[g_perm[I]][g_perm[J]];
+            g_nnz_lu += g_fill[I][J];
+        }
+}
+
+static void ora_init(void) { pthread_once(&g_once, ora_init_tables); }
+
+void ora_jacobian(const double *rk, const double *y, double *jac)
+{
+    ora_init();
+    memset(jac, 0, sizeof(double) * NS * NS);
+    for (int e = 0; e < g_nje; e++) {
+        double acc = 0.0;
+        for (int t = g_je_start[e]; t < g_je_start[e + 1]; t++) {
+            const jterm *q = &g_jt[t];
+            double dw = rk[q->r];
+            if (q->other >= 0) dw = dw * y[q->other];
+            double term = q->c * dw;
+            acc = (t == g_je_start[e]) ? term : acc + term;
+        }
+        jac[g_je_i[e] * NS + g_je_j[e]] = acc;
+    }
+}
+
+int ora_lu_order(int *perm, int *nnz_lu)
+{
+    ora_init();
+    for (int k = 0; k < NS; k++) perm[k] = g_perm[k];
+    if (nnz_lu) *nnz_lu = g_nnz_lu;
+    return NS;
+}
+
+/* ------------------------------------------------------------------ LU
+ * Numeric refactor with Sparse1.3's arithmetic (spFactor.c:336-401): column by
+ * column, U(k,J) = D_k * (1/L_kk) for k<J in row order, each followed by
+ * D_i -= U(k,J) L(i,k) for i>k in row order; the diagonal keeps 1/pivot.
+ * Solve as spSolve.c:183-218: forward with reciprocal pivots (skipping zero
+ * entries), backward with unit-diagonal U.  Works on internal indices. */
+typedef struct { double v[NS][NS]; } lumat;
+
+static int ora_lu_factor(lumat *m)
+{
+    double D[NS];
+    if (m->v[0][0] == 0.0) return 1;
+    m->v[0][0] = 1.0 / m->v[0][0];
+    for (int J = 1; J < NS; J++) {
+        for (int i = 0; i < NS; i++) D[i] = g_fill[i][J] ? m->v[i][J] : 0.0;
+        for (int K = 0; K < J; K++) {
+            if (!g_fill[K][J]) continue;
+            double u = D[K] * m->v[K][K];
+            m->v[K][J] = u;
+            for (int I = K + 1; I < NS; I++)
+                if (g_fill[I][K]) D[I] -= u * m->v[I][K];
+        }
+        for (int I = J + 1; I < NS; I++)
+            if (g_fill[I][J]) m->v[I][J] = D[I];
+        if (D[J] == 0.0) return J + 1;
+        m->v[J][J] = 1.0 / D[J];
+    }
+    return 0;
+}
+
+static void ora_lu_solve(const lumat *m, const double *rhs, double *x)
+{
+    double c[NS];
+    for (int I = 0; I < NS; I++) c[I] = rhs[g_perm[I]];
+    for (int I = 0; I < NS; I++) {
+        double t = c[I];
+        if (t != 0.0) {
+            t *= m->v[I][I];
+            c[I] = t;
+            for (int R = I + 1; R < NS; R++)
+                if (g_fill[R][I]) c[R] -= t * m->v[R][I];
+        }
+    }
+    for (int I = NS - 1; I >= 0; I--) {
+        double t = c[I];
+        for (int J = I + 1; J < NS; J++)
+            if (g_fill[I][J]) t -= m->v[I][J] * c[J];
+        c[I] = t;
+    }
+    for (int I = 0; I < NS; I++) x[g_perm[I]] = c[I];
+}
+
+/* load a masked species-indexed matrix into internal order */
+static void ora_lu_load(lumat *m, const double *a, const unsigned char *mask)
+{
+    for (int I = 0; I < NS; I++)
+        for (int J = 0; J < NS; J++) {
+            int i = g_perm[I], j = g_perm[J];
+            m->v[I][J] = (g_fill[I][J] && mask[i * NS + j]) ? a[i * NS + j] : 0.0;
+        }
+}
+
+int ora_lu_solve_dense(const double *a, const unsigned char *mask, const double *b, double *x)
+{
+    ora_init();
+    lumat m;
+    ora_lu_load(&m, a, mask);
+    int rc = ora_lu_factor(&m);
+    if (rc) return rc;
+    ora_lu_solve(&m, b, x);
+    return 0;
+}
+
+/* external Sparse1.3 hooks */
+static ora_sp_create_fn g_sp_create;
+static ora_sp_load_factor_fn g_sp_load_factor;
+static ora_sp_solve_fn g_sp_solve;
+
+void ora_use_external_sparse(ora_sp_create_fn c, ora_sp_load_factor_fn lf, ora_sp_solve_fn s)
+{
+    g_sp_create = c;
+    g_sp_load_factor = lf;
+    g_sp_solve = s;
+}
+
+/* ------------------------------------------------------------------ blocks
+ * AllocIndex, chem_spack_utils.f90:93-125: nob = N/block_size (1 if N <=
+ * block_size), the remainder added +1 to blocks 1,2,... in turn. */
+int ora_alloc_index(int block_size, int m1, int ia, int iz, int ja, int jz,
+                    int32_t *block_end, int max_nob)
+{
+    long ncell = (long)(m1 - 2) * (iz - ia + 1) * (jz - ja + 1);
+    if (ncell <= 0 || block_size <= 0) return -1;
+    int nob = (block_size >= ncell) ? 1 : (int)(ncell / block_size);
+    if (!block_end) return nob;
+    if (nob > max_nob) return -2;
+    long rest = ncell;
+    for (int b = 0; b < nob; b++) { block_end[b] = (block_size >= ncell) ? (int)ncell : block_size; }
+    rest = ncell - (long)nob * ((block_size >= ncell) ? ncell : block_size);
+    while (rest > 0)
+        for (int b = 0; b < nob && rest > 0; b++) { block_end[b]++; rest--; }
+    return nob;
+}
+
+/* ------------------------------------------------------------------ driver */
+typedef struct {
+    const brams_chem_args *a;
+    int nob;
+    long *bstart;                /* first global cell of each block */
+    unsigned char mask[NS * NS]; /* value-based sparsity mask of block 1 */
+    int64_t *attempts, *substeps;
+    int *failed;
+    atomic_int next;
+    int has_sparse;              /* external Sparse1.3 in use */
+    long sp_handle;
+    long *sp_elem;
+    int *sp_ipos, *sp_jpos;
+    int sp_nnz;
+} drv_t;
+
+typedef struct {
+    int n;
+    long off[128];
+    double press[128], temp[128], vapp[128], volmol[128], volmol_i[128];
+    double y[128][NS], ynew[128][NS], rk[128][NR], jph[128][NP];
+    double f0[128][NS], k1[128][NS], k2[128][NS], k3[128][NS], k4[128][NS], ftmp[NS];
+    double err[128];
+    double jac[128][NS * NS];
+    lumat lu;
+} blk_t;
+
+static int64_t *g_last_attempts, *g_last_substeps;
+static int g_last_nob;
+
+static void cell_offset(const brams_chem_args *a, long g, long *off)
+{
+    long nk = a->m1 - 2, ni = a->iz - a->ia + 1;
+    long k = 2 + g % nk;
+    long i = a->ia + (g / nk) % ni;
+    long j = a->ja + g / (nk * ni);
+    *off = (k - 1) + (long)a->m1 * (i - 1) + (long)a->m1 * a->m2 * (j - 1);
+}
+
+/* gather and unit conversion, chem_spack_rodas3_dyndt.f90:226-354 */
+static void gather(const brams_chem_args *a, blk_t *B)
+{
+    const float cp = a->cp, cpor = a->cpor, p00 = a->p00;
+    for (int c = 0; c < B->n; c++) {
+        long o = B->off[c];
+        float exner = (a->pp[o] + a->pi0[o]) / cp;            /* REAL arithmetic */
+        double press = (double)exner;
+        B->temp[c] = (double)a->theta[o] * press;
+        press = pow(press, (double)cpor) * (double)p00;
+        B->press[c] = press;
+        B->vapp[c] = (double)a->rv[o];
+        B->volmol[c] = (6.02e23 * 1e-15 * 28.96) * press / (8.314 * B->temp[c]);
+        B->volmol_i[c] = 1.0 / B->volmol[c];
+        for (int t = 0; t < a->n_not_transported; t++) {
+            int n = a->no_transp_chem_index[t] - 1;
+            B->y[c][n] = (double)a->sc_p[n][o];
+        }
+        for (int t = 0; t < a->n_transported; t++) {
+            int n = a->transp_chem_index[t] - 1;
+            if (a->split_method == BRAMS_SPLIT_PARALLEL && a->n_dyn_chem > 1) {
+                float start = a->sc_p[n][o] - a->sc_t_dyn[n][o] * (float)a->n_dyn_chem * a->dtlt;
+                B->y[c][n] = (double)start * B->volmol[c] / (double)a->weight[n];
+            } else {
+                double v = (double)a->sc_p[n][o] * B->volmol[c] / (double)a->weight[n];
+                B->y[c][n] = (v > 0.0) ? v : 0.0;
+            }
+        }
+        for (int p = 0; p < NP; p++) B->jph[c][p] = a->jphoto[p + (long)NP * o];
+    }
+}
+
+/* write-back, chem_spack_rodas3_dyndt.f90:713-798 */
+static void scatter(const brams_chem_args *a, blk_t *B)
+{
+    for (int c = 0; c < B->n; c++) {
+        long o = B->off[c];
+        for (int t = 0; t < a->n_transported; t++) {
+            int n = a->transp_chem_index[t] - 1;
+            double conv = B->y[c][n] * (double)a->weight[n] * B->volmol_i[c];
+            if (a->split_method == BRAMS_SPLIT_PARALLEL && a->n_dyn_chem > 1) {
+                float base = a->sc_t_dyn[n][o] * (float)a->n_dyn_chem * a->dtlt;
+                float v = (float)((double)base + conv);
+                a->sc_p[n][o] = (v > 0.0f) ? v : 0.0f;
+            } else if (a->split_method == BRAMS_SPLIT_PARALLEL) {
+                double dti = 1.0 / (double)a->dtlt;
+                a->sc_t[n][o] = (float)((double)a->sc_t[n][o] + (conv - (double)a->sc_p[n][o]) * dti);
+            } else {
+                float v = (float)conv;
+                a->sc_p[n][o] = (v > 0.0f) ? v : 0.0f;
+            }
+        }
+        for (int t = 0; t < a->n_not_transported; t++) {
+            int n = a->no_transp_chem_index[t] - 1;
+            float v = (float)B->y[c][n];
+            a->sc_p[n][o] = (v > 0.0f) ? v : 0.0f;
+        }
+    }
+}
+
+static void set_mask_from_cell0(drv_t *D, const double *jac0, double igdt)
+{
+    /* mask = entries of DLmat(cell 1) that are nonzero, rodas3:470-482 */
+    for (int i = 0; i < NS; i++)
+        for (int j = 0; j < NS; j++) {
+            double v = (i == j) ? igdt - jac0[i * NS + j] : -jac0[i * NS + j];
+            D->mask[i * NS + j] = (v != 0.0);
+        }
+}
+
+/* RODAS3 on one block, chem_spack_rodas3_dyndt.f90:413-705 */
+static int integrate_block(drv_t *D, blk_t *B, int b)
+{
+    const brams_chem_args *a = D->a;
+    const double c83 = 8.0 / 3.0;
+    const float tn = a->dtlt * (float)a->n_dyn_chem;     /* REAL product */
+    double dt = a->last_accepted_dt[b];
+    const double dt_min = fmax(1.0, 1.0e-2 * (double)tn);
+    const double dt_max = (double)tn;
+    double time_c = 0.0;
+    const double time_f = (double)tn;
+    int failed = 0;
+    int64_t natt = 0, nsub = 0;
+
+    for (int c = 0; c < B->n; c++)
+        ora_kinetic(B->temp[c], B->press[c], B->vapp[c], B->jph[c], B->rk[c]);
+
+    while (time_c + 1.0e-8 < time_f) {
+        for (int c = 0; c < B->n; c++) {
+            ora_jacobian(B->rk[c], B->y[c], B->jac[c]);
+            ora_fexchem(B->rk[c], B->y[c], B->f0[c]);
+        }
+        for (;;) {
+            natt++;
+            const double igdt = 1.0 / (0.5 * dt);
+            if (b == 0) {
+                set_mask_from_cell0(D, B->jac[0], igdt);
+                if (D->has_sparse) {
+                    /* new matrix + ordering on cell-1 values each attempt, rodas3:470-506 */
+                    int nz = 0;
+                    double vals[NS * NS];
+                    for (int i = 0; i < NS; i++)
+                        for (int j = 0; j < NS; j++)
+                            if (D->mask[i * NS + j]) {
+                                D->sp_ipos[nz] = i + 1;
+                                D->sp_jpos[nz] = j + 1;
+                                vals[nz] = (i == j) ? igdt - B->jac[0][i * NS + j] : -B->jac[0][i * NS + j];
+                                nz++;
+                            }
+                    D->sp_nnz = nz;
+                    D->sp_handle = g_sp_create(NS, nz, D->sp_ipos, D->sp_jpos, D->sp_elem);
+                    g_sp_load_factor(D->sp_handle, nz, D->sp_elem, vals);
+                }
+            }
+            const double dti = 1.0 / dt;
+            for (int c = 0; c < B->n; c++) {
+                double mat[NS * NS];
+                for (int i = 0; i < NS; i++)
+                    for (int j = 0; j < NS; j++)
+                        mat[i * NS + j] = (i == j) ? igdt - B->jac[c][i * NS + j] : -B->jac[c][i * NS + j];
+                double rhs[NS];
+                if (D->has_sparse) {
+                    double vals[NS * NS];
+                    for (int z = 0; z < D->sp_nnz; z++)
+                        vals[z] = mat[(D->sp_ipos[z] - 1) * NS + D->sp_jpos[z] - 1];
+                    if (g_sp_load_factor(D->sp_handle, D->sp_nnz, D->sp_elem, vals) != 0) failed = 1;
+                } else {
+                    ora_lu_load(&B->lu, mat, D->mask);
+                    if (ora_lu_factor(&B->lu) != 0) failed = 1;
+                }
+#define SOLVE(in, out) do { if (D->has_sparse) { memcpy(rhs, in, sizeof rhs); g_sp_solve(D->sp_handle, rhs, out); } \
+                            else ora_lu_solve(&B->lu, in, out); } while (0)
+                double *y0 = B->y[c], *k1 = B->k1[c], *k2 = B->k2[c], *k3 = B->k3[c], *k4 = B->k4[c];
+                double *yn = B->ynew[c], bb[NS];
+                /* stage 1, rodas3:523-527 */
+                SOLVE(B->f0[c], k1);
+                /* stage 2, rodas3:534-538 */
+                for (int s = 0; s < NS; s++) bb[s] = (4.0 * dti) * k1[s] + B->f0[c][s];
+                SOLVE(bb, k2);
+                /* stage 3 with clipping, rodas3:542-566 */
+                for (int s = 0; s < NS; s++) {
+                    yn[s] = y0[s] + 2.0 * k1[s];
+                    if (yn[s] < 0.0) { yn[s] = 0.0; k1[s] = 0.5 * (yn[s] - y0[s]); }
+                }
+                ora_fexchem(B->rk[c], yn, B->ftmp);
+                for (int s = 0; s < NS; s++) bb[s] = B->ftmp[s] + dti * (k1[s] - k2[s]);
+                SOLVE(bb, k3);
+                /* stage 4 with clipping, rodas3:573-599 */
+                for (int s = 0; s < NS; s++) {
+                    yn[s] = y0[s] + 2.0 * k1[s] + k3[s];
+                    if (yn[s] < 0.0) { yn[s] = 0.0; k3[s] = (yn[s] - y0[s]) - 2.0 * k1[s]; }
+                }
+                ora_fexchem(B->rk[c], yn, B->ftmp);
+                for (int s = 0; s < NS; s++) bb[s] = B->ftmp[s] + dti * (k1[s] - k2[s] - c83 * k3[s]);
+                SOLVE(bb, k4);
+#undef SOLVE
+            }
+            /* solution and error, rodas3:609-648 */
+            double max_err = 0.0;
+            int all_accepted = 1;
+            for (int c = 0; c < B->n; c++) {
+                double e = 0.0;
+                for (int s = 0; s < NS; s++) {
+                    double v = B->y[c][s] + 2.0 * B->k1[c][s] + B->k3[c][s] + B->k4[c][s];
+                    v = fmax(v, 0.0);
+                    B->ynew[c][s] = v;
+                    double tol = a->atol[s] + a->rtol[s] * fmax(fabs(B->y[c][s]), fabs(v));
+                    double q = B->k4[c][s] / tol;
+                    e = e + q * q;
+                }
+                B->err[c] = fmax(1.0e-15, sqrt(e / NS));
+                if (!(B->err[c] == B->err[c]) || isinf(B->err[c])) failed = 1;
+                if (B->err[c] - 1.0e-8 > 1.0) all_accepted = 0;
+                if (c == 0 || B->err[c] > max_err) max_err = B->err[c];
+            }
+            /* controller, rodas3:650-700 */
+            double fac = fmin(6.0, fmax(0.2, 0.9 / pow(max_err, 1.0 / 3.0)));
+            double dt_new = dt * fac;
+            dt_new = fmax(dt_min, fmin(dt_max, dt_new));
+            if (failed) { time_c = time_f; break; }
+            if (!all_accepted && dt_new > dt_min) {
+                dt = dt_new;
+                continue;
+            }
+            time_c = time_c + dt;
+            dt = fmin(dt_new, time_f - time_c);
+            if (time_c < time_f) a->last_accepted_dt[b] = dt_new;
+            for (int c = 0; c < B->n; c++) memcpy(B->y[c], B->ynew[c], sizeof(double) * NS);
+            nsub++;
+            break;
+        }
+    }
+    D->attempts[b] = natt;
+    D->substeps[b] = nsub;
+    D->failed[b] = failed;
+    return failed;
+}
+
+static void run_block(drv_t *D, blk_t *B, int b)
+{
+    B->n = D->a->block_end[b];
+    for (int c = 0; c < B->n; c++) cell_offset(D->a, D->bstart[b] + c, &B->off[c]);
+    gather(D->a, B);
+    integrate_block(D, B, b);
+    scatter(D->a, B);
+}
+
+static void *worker(void *arg)
+{
+    drv_t *D = (drv_t *)arg;
+    blk_t *B = (blk_t *)malloc(sizeof(blk_t));
+    for (;;) {
+        int b = atomic_fetch_add(&D->next, 1);
+        if (b >= D->nob) break;
+        run_block(D, B, b);
+    }
+    free(B);
+    return NULL;
+}
+
+int ora_rodas3_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
+                     unsigned char *mask_out)
+{
+    ora_init();
+    if (a->nspecies != NS || a->nr != NR || a->nr_photo != NP) return -1;
+    if (a->photoj_method == BRAMS_PHOTOJ_LUT) return -2;
+    if (a->maxblock_size > 128) return -3;
+    drv_t *D = (drv_t *)calloc(1, sizeof(drv_t));
+    D->a = a;
+    D->nob = a->nob;
+    D->bstart = (long *)malloc(sizeof(long) * (a->nob + 1));
+    D->attempts = (int64_t *)calloc(a->nob, sizeof(int64_t));
+    D->substeps = (int64_t *)calloc(a->nob, sizeof(int64_t));
+    D->failed = (int *)calloc(a->nob, sizeof(int));
+    D->bstart[0] = 0;
+    for (int b = 0; b < a->nob; b++) D->bstart[b + 1] = D->bstart[b] + a->block_end[b];
+    D->has_sparse = (g_sp_create && g_sp_load_factor && g_sp_solve);
+    D->sp_elem = (long *)malloc(sizeof(long) * NS * NS);
+    D->sp_ipos = (int *)malloc(sizeof(int) * NS * NS);
+    D->sp_jpos = (int *)malloc(sizeof(int) * NS * NS);
+
+    /* block 1 first: it fixes the mask (and, in the reference, the order) */
+    blk_t *B = (blk_t *)malloc(sizeof(blk_t));
+    run_block(D, B, 0);
+    free(B);
+    if (mask_out) memcpy(mask_out, D->mask, NS * NS);
+
+    atomic_store(&D->next, 1);
+    if (nthreads <= 0) nthreads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (D->has_sparse) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, worker, D);
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+
+    int nfail = 0;
+    int64_t att = 0, sub = 0, cells = 0;
+    for (int b = 0; b < a->nob; b++) {
+        nfail += D->failed[b];
+        att += D->attempts[b];
+        sub += D->substeps[b];
+        cells += a->block_end[b];
+    }
+    if (st) {
+        memset(st, 0, sizeof *st);
+        st->cells = cells;
+        st->blocks = a->nob;
+        st->substeps = sub;
+        st->attempts = att;
+        st->failed_blocks = nfail;
+        int nz = 0;
+        for (int i = 0; i < NS * NS; i++) nz += D->mask[i];
+        st->mask_nnz = nz;
+    }
+    free(g_last_attempts);
+    free(g_last_substeps);
+    g_last_attempts = D->attempts;
+    g_last_substeps = D->substeps;
+    g_last_nob = a->nob;
+    free(D->failed);
+    free(D->bstart);
+    free(D->sp_elem);
+    free(D->sp_ipos);
+    free(D->sp_jpos);
+    free(D);
+    return nfail;
+}
+
+int ora_block_counts(int64_t *attempts, int64_t *substeps, int max_nob)
+{
+    int n = g_last_nob < max_nob ? g_last_nob : max_nob;
+    for (int b = 0; b < n; b++) {
+        if (attempts) attempts[b] = g_last_attempts[b];
+        if (substeps) substeps[b] = g_last_substeps[b];
+    }
+    return g_last_nob;
+}

@@ -1,0 +1,66 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * CPU restatement of the reference's CCATT RODAS3 chemistry step
+ * (src/brams/ccatt/chem_spack_rodas3_dyndt.f90) and of the RELACS_TUV
+ * mechanism kernels it calls.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load it, as the checker; the product path
+ * (brams_amd/) never calls it.
+ *
+ * Pinning (see tests/):
+ *   ora_kinetic / ora_fexchem / ora_jacobian  bit-for-bit against the
+ *       reference's own compiled kernels (oracle/_ref) at random states;
+ *   ora_lu_*        against the reference's compiled Sparse1.3 and against
+ *       numpy.linalg.solve (residuals);
+ *   ora_alloc_index against the reference's compiled AllocIndex;
+ *   ora_rodas3_dyndt end to end against the same driver run on the
+ *       reference's Sparse1.3 (ora_use_external_sparse), plus the closed-form
+ *       checks of the Rodas3 tableau (order 3 on y'=Ay vs expm) and of the
+ *       step-size controller.
+ */
+#ifndef BRAMS_ORACLE_H
+#define BRAMS_ORACLE_H
+#include <stdint.h>
+#include "../include/brams_chem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One cell's rate coefficients rk[128] from T (K), p (Pa), rv (kg/kg), J[17]. */
+void ora_kinetic(double temp, double press, double vapp, const double *jphoto, double *rk);
+/* Net production f[47] = S w(rk, y). */
+void ora_fexchem(const double *rk, const double *y, double *f);
+/* Jacobian df/dy, dense row-major jac[i*47+j] = d f_i / d y_j. */
+void ora_jacobian(const double *rk, const double *y, double *jac);
+
+/* Fixed pivot order used by the oracle's LU: perm[k] = species pivoted at step k. */
+int ora_lu_order(int *perm, int *nnz_lu);
+/* Factor the masked matrix a (row-major 47x47, mask 0/1) and solve a x = b. */
+int ora_lu_solve_dense(const double *a, const unsigned char *mask, const double *b, double *x);
+
+/* Block partition, AllocIndex (chem_spack_utils.f90:39-183); returns nob. */
+int ora_alloc_index(int block_size, int m1, int ia, int iz, int ja, int jz,
+                    int32_t *block_end, int max_nob);
+
+/* The whole step.  nthreads<=0: use all host cores.  mask_out (47*47, may be
+ * NULL) receives the value-based sparsity mask blocks 2..nob used. */
+int ora_rodas3_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
+                     unsigned char *mask_out);
+
+/* Optional per-block trace of block b: attempts, sub-steps. */
+int ora_block_counts(int64_t *attempts, int64_t *substeps, int max_nob);
+
+/* Route the LU through an external Sparse1.3 build (the reference's, via
+ * oracle/ref_sparse_harness.c).  NULL pointers restore the oracle's own LU.
+ * With an external LU the step runs single-threaded, exactly as the reference
+ * drives Sparse (chem_spack_rodas3_dyndt.f90:470-603). */
+typedef long (*ora_sp_create_fn)(int n, int nnz, const int *ipos, const int *jpos, long *elements);
+typedef int (*ora_sp_load_factor_fn)(long h, int nnz, const long *elements, const double *vals);
+typedef void (*ora_sp_solve_fn)(long h, double *rhs, double *x);
+void ora_use_external_sparse(ora_sp_create_fn c, ora_sp_load_factor_fn lf, ora_sp_solve_fn s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
