@@ -1,0 +1,114 @@
+"""ctypes view of the C-ABI in include/brams_chem.h (libbrams_chem.so).
+
+Loading fails loudly when the HIP library has not been built: there is no CPU
+fallback anywhere in the product path.
+"""
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libbrams_chem.so")
+
+NSPECIES, NREACT, NPHOTO = 47, 128, 17
+SPLIT = {"SYMMETRIC": 0, "SEQUENTIAL": 1, "PARALLEL": 2}
+PHOTOJ = {"FAST-TUV": 0, "FAST-JX": 1, "LUT": 2}
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+
+# symbols include/brams_chem.h declares
+EXPORTS = ["brams_chem_init", "brams_chem_rodas3_dyndt", "brams_chem_rodas3_dyndt_device",
+           "brams_chem_alloc_index", "brams_chem_get_stats", "brams_chem_last_error",
+           "brams_chem_finalize"]
+
+
+class ChemArgs(C.Structure):
+    """brams_chem_args"""
+    _fields_ = [
+        ("m1", C.c_int32), ("m2", C.c_int32), ("m3", C.c_int32),
+        ("ia", C.c_int32), ("iz", C.c_int32), ("ja", C.c_int32), ("jz", C.c_int32),
+        ("dtlt", C.c_float), ("n_dyn_chem", C.c_int32), ("split_method", C.c_int32),
+        ("photoj_method", C.c_int32),
+        ("cp", C.c_float), ("cpor", C.c_float), ("p00", C.c_float),
+        ("pp", _fp), ("pi0", _fp), ("theta", _fp), ("rv", _fp),
+        ("jphoto", _dp),
+        ("nspecies", C.c_int32), ("nr", C.c_int32), ("nr_photo", C.c_int32),
+        ("weight", _fp),
+        ("n_transported", C.c_int32), ("n_not_transported", C.c_int32),
+        ("transp_chem_index", _ip), ("no_transp_chem_index", _ip),
+        ("sc_p", C.POINTER(_fp)), ("sc_t", C.POINTER(_fp)), ("sc_t_dyn", C.POINTER(_fp)),
+        ("nob", C.c_int32), ("maxblock_size", C.c_int32),
+        ("block_end", _ip), ("last_accepted_dt", _dp),
+        ("atol", _dp), ("rtol", _dp),
+    ]
+
+
+class ChemDevArgs(C.Structure):
+    """brams_chem_dev_args"""
+    _fields_ = [
+        ("m1", C.c_int32), ("m2", C.c_int32), ("m3", C.c_int32),
+        ("ia", C.c_int32), ("iz", C.c_int32), ("ja", C.c_int32), ("jz", C.c_int32),
+        ("dtlt", C.c_float), ("n_dyn_chem", C.c_int32), ("split_method", C.c_int32),
+        ("photoj_method", C.c_int32),
+        ("cp", C.c_float), ("cpor", C.c_float), ("p00", C.c_float),
+        ("pp", C.c_void_p), ("pi0", C.c_void_p), ("theta", C.c_void_p), ("rv", C.c_void_p),
+        ("jphoto", C.c_void_p),
+        ("weight", C.c_void_p), ("transported", C.c_void_p),
+        ("sc_p", C.c_void_p), ("sc_t", C.c_void_p), ("sc_t_dyn", C.c_void_p),
+        ("nob", C.c_int32),
+        ("last_accepted_dt", C.c_void_p),
+        ("atol", C.c_void_p), ("rtol", C.c_void_p),
+        ("stream", C.c_void_p),
+    ]
+
+
+class ChemStats(C.Structure):
+    """brams_chem_stats"""
+    _fields_ = [("cells", C.c_int64), ("blocks", C.c_int64), ("substeps", C.c_int64),
+                ("attempts", C.c_int64), ("failed_blocks", C.c_int64),
+                ("mask_nnz", C.c_int32), ("kernel_ms", C.c_float),
+                ("cell_substeps", C.c_int64), ("cell_attempts", C.c_int64),
+                ("block1_ms", C.c_float), ("main_ms", C.c_float), ("main_cells", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ChemError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libbrams_chem.so; raise if it is missing (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ChemError("libbrams_chem.so is not built (run __graft_entry__.build()); "
+                            "there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        L.brams_chem_init.argtypes = [C.c_int]
+        L.brams_chem_rodas3_dyndt.argtypes = [C.POINTER(ChemArgs)]
+        L.brams_chem_rodas3_dyndt_device.argtypes = [C.POINTER(ChemDevArgs)]
+        L.brams_chem_alloc_index.argtypes = [C.c_int32] * 6 + [_ip, C.c_int32]
+        L.brams_chem_get_stats.argtypes = [C.POINTER(ChemStats)]
+        L.brams_chem_last_error.restype = C.c_char_p
+        L.brams_chem_finalize.argtypes = []
+        _lib = L
+    return _lib
+
+
+def check(rc, what="call"):
+    """Raise on a fatal status; return the count of failed blocks otherwise."""
+    if rc < 0:
+        raise ChemError("%s failed (%d): %s" % (what, rc, lib().brams_chem_last_error().decode()))
+    return rc
+
+
+def stats():
+    s = ChemStats()
+    check(lib().brams_chem_get_stats(C.byref(s)), "brams_chem_get_stats")
+    return s

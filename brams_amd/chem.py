@@ -1,0 +1,177 @@
+"""Host-side mirror of the reference's chemistry call, over the C-ABI.
+
+chem_rodas3_dyndt() keeps the argument names, meaning and units of
+mod_chem_spack_rodas3_dyndt::chem_rodas3_dyndt
+(src/brams/ccatt/chem_spack_rodas3_dyndt.f90:41-131) for the arguments the
+RELACS_TUV path reads; it updates chem1_sc_p (or sc_t) and last_accepted_dt in
+place like the Fortran routine.  DeviceStep runs the same step on arrays that
+already live in HBM (torch tensors are used only as device memory).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .mechanism import load_mechanism
+
+_fp = C.POINTER(C.c_float)
+
+
+def _f32(a):
+    a = np.asarray(a)
+    if a.dtype != np.float32 or not a.flags["F_CONTIGUOUS"]:
+        raise abi.ChemError("expected a Fortran-ordered float32 array")
+    return a.ctypes.data_as(_fp)
+
+
+def alloc_index(m1, ia, iz, ja, jz, block_size=64):
+    """AllocIndex block sizes (chem_spack_utils.f90:39-183), through the C-ABI."""
+    L = abi.lib()
+    nob = abi.check(L.brams_chem_alloc_index(block_size, m1, ia, iz, ja, jz, None, 0), "alloc_index")
+    out = np.zeros(nob, np.int32)
+    abi.check(L.brams_chem_alloc_index(block_size, m1, ia, iz, ja, jz,
+                                       out.ctypes.data_as(C.POINTER(C.c_int32)), nob), "alloc_index")
+    return out
+
+
+def chem_rodas3_dyndt(m1, m2, m3, ia, iz, ja, jz, dtlt, pp, pi0, theta, rv, nspecies, nr, nr_photo,
+                      weight, PhotojMethod, transp_chem_index, no_transp_chem_index, chem1_sc_p,
+                      nob, maxblock_size, block_end, last_accepted_dt, atol, rtol, N_DYN_CHEM,
+                      split_method, cp, cpor, p00, jphoto, sc_t=None, sc_t_dyn=None, device=0):
+    """One chemistry step on one grid; arrays are updated in place.
+
+    Returns the number of blocks with a numerical failure (the reference would
+    abort there); raises abi.ChemError on invalid arguments or a HIP error."""
+    L = abi.lib()
+    abi.check(L.brams_chem_init(device), "brams_chem_init")
+    keep = []
+    a = abi.ChemArgs()
+    a.m1, a.m2, a.m3, a.ia, a.iz, a.ja, a.jz = m1, m2, m3, ia, iz, ja, jz
+    a.dtlt = dtlt
+    a.n_dyn_chem = N_DYN_CHEM
+    a.split_method = abi.SPLIT[split_method.strip()]
+    a.photoj_method = abi.PHOTOJ[PhotojMethod.strip()]
+    a.cp, a.cpor, a.p00 = cp, cpor, p00
+    a.pp, a.pi0, a.theta, a.rv = _f32(pp), _f32(pi0), _f32(theta), _f32(rv)
+    jph = np.asarray(jphoto)
+    if jph.dtype != np.float64 or not jph.flags["F_CONTIGUOUS"]:
+        raise abi.ChemError("jphoto must be Fortran-ordered float64")
+    a.jphoto = jph.ctypes.data_as(C.POINTER(C.c_double))
+    a.nspecies, a.nr, a.nr_photo = nspecies, nr, nr_photo
+    w = np.ascontiguousarray(weight, np.float32); keep.append(w)
+    a.weight = w.ctypes.data_as(_fp)
+    ti = np.ascontiguousarray(transp_chem_index, np.int32); keep.append(ti)
+    ni = np.ascontiguousarray(no_transp_chem_index, np.int32); keep.append(ni)
+    a.n_transported, a.n_not_transported = len(ti), len(ni)
+    a.transp_chem_index = ti.ctypes.data_as(C.POINTER(C.c_int32))
+    a.no_transp_chem_index = ni.ctypes.data_as(C.POINTER(C.c_int32))
+    tab = (_fp * nspecies)(*[_f32(x) for x in chem1_sc_p]); keep.append(tab)
+    a.sc_p = C.cast(tab, C.POINTER(_fp))
+    if sc_t is not None:
+        tt = (_fp * nspecies)(*[_f32(x) for x in sc_t]); keep.append(tt)
+        a.sc_t = C.cast(tt, C.POINTER(_fp))
+    if sc_t_dyn is not None:
+        td = (_fp * nspecies)(*[_f32(x) for x in sc_t_dyn]); keep.append(td)
+        a.sc_t_dyn = C.cast(td, C.POINTER(_fp))
+    be = np.ascontiguousarray(block_end, np.int32); keep.append(be)
+    a.nob, a.maxblock_size = nob, maxblock_size
+    a.block_end = be.ctypes.data_as(C.POINTER(C.c_int32))
+    if last_accepted_dt.dtype != np.float64 or not last_accepted_dt.flags["C_CONTIGUOUS"]:
+        raise abi.ChemError("last_accepted_dt must be a contiguous float64 array")
+    a.last_accepted_dt = last_accepted_dt.ctypes.data_as(C.POINTER(C.c_double))
+    at = np.ascontiguousarray(atol, np.float64); keep.append(at)
+    rt = np.ascontiguousarray(rtol, np.float64); keep.append(rt)
+    a.atol = at.ctypes.data_as(C.POINTER(C.c_double))
+    a.rtol = rt.ctypes.data_as(C.POINTER(C.c_double))
+    return abi.check(L.brams_chem_rodas3_dyndt(C.byref(a)), "brams_chem_rodas3_dyndt")
+
+
+def run_workload_host(wl, atol, rtol, split_method="SYMMETRIC", n_dyn_chem=1, sc_t=None,
+                      sc_t_dyn=None):
+    """Convenience: run chem_rodas3_dyndt on a brams_amd.workload dict (copies)."""
+    mech = load_mechanism()
+    g = wl["grid"]
+    sc_p = [np.array(x, copy=True, order="F") for x in wl["sc_p"]]
+    last_dt = np.array(wl["last_accepted_dt"], dtype=np.float64, copy=True)
+    tidx = [i + 1 for i, s in enumerate(mech.species) if s.transported]
+    nidx = [i + 1 for i, s in enumerate(mech.species) if not s.transported]
+    sct = None if sc_t is None else [np.array(x, copy=True, order="F") for x in sc_t]
+    rc = chem_rodas3_dyndt(g.m1, g.m2, g.m3, g.ia, g.iz, g.ja, g.jz, g.dtlt, wl["pp"], wl["pi0"],
+                           wl["theta"], wl["rv"], mech.nspecies, mech.nr, 17,
+                           [s.weight for s in mech.species], "FAST-TUV", tidx, nidx, sc_p,
+                           len(wl["block_end"]), int(max(wl["block_end"])), wl["block_end"],
+                           last_dt, atol, rtol, n_dyn_chem, split_method, 1004.0, 1004.0 / 287.0,
+                           1.0e5, wl["jphoto"], sc_t=sct, sc_t_dyn=sc_t_dyn)
+    return sc_p, last_dt, rc, sct
+
+
+class DeviceStep:
+    """A chemistry step whose arrays stay resident in HBM across calls.
+
+    Inputs are uploaded once; step() runs brams_chem_rodas3_dyndt_device on the
+    resident state (sc_p and last_accepted_dt are updated in place on the
+    device, as a time loop would)."""
+
+    def __init__(self, wl, atol, rtol, device=0, split_method="SYMMETRIC", n_dyn_chem=1):
+        import torch
+        self.torch = torch
+        L = abi.lib()
+        abi.check(L.brams_chem_init(device), "brams_chem_init")
+        mech = load_mechanism()
+        g = wl["grid"]
+        self.grid = g
+        dev = torch.device("cuda", device)
+        self.dev = dev
+
+        def up(a, dt):
+            return torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=dt).ravel(order="F"))).to(dev)
+
+        self.pp = up(wl["pp"], np.float32)
+        self.pi0 = up(wl["pi0"], np.float32)
+        self.theta = up(wl["theta"], np.float32)
+        self.rv = up(wl["rv"], np.float32)
+        self.jphoto = up(wl["jphoto"], np.float64)
+        self.sc_p0 = torch.stack([up(x, np.float32) for x in wl["sc_p"]])       # [47, n3d]
+        self.sc_p = self.sc_p0.clone()
+        self.weight = torch.tensor([s.weight for s in mech.species], dtype=torch.float32, device=dev)
+        self.transported = torch.tensor([1 if s.transported else 0 for s in mech.species],
+                                        dtype=torch.int32, device=dev)
+        self.last_dt0 = torch.tensor(np.asarray(wl["last_accepted_dt"], np.float64), device=dev)
+        self.last_dt = self.last_dt0.clone()
+        self.atol = torch.tensor(np.asarray(atol, np.float64), device=dev)
+        self.rtol = torch.tensor(np.asarray(rtol, np.float64), device=dev)
+        a = abi.ChemDevArgs()
+        a.m1, a.m2, a.m3, a.ia, a.iz, a.ja, a.jz = g.m1, g.m2, g.m3, g.ia, g.iz, g.ja, g.jz
+        a.dtlt = g.dtlt
+        a.n_dyn_chem = n_dyn_chem
+        a.split_method = abi.SPLIT[split_method]
+        a.photoj_method = 0
+        a.cp, a.cpor, a.p00 = 1004.0, 1004.0 / 287.0, 1.0e5
+        a.pp, a.pi0, a.theta, a.rv = (self.pp.data_ptr(), self.pi0.data_ptr(),
+                                      self.theta.data_ptr(), self.rv.data_ptr())
+        a.jphoto = self.jphoto.data_ptr()
+        a.weight = self.weight.data_ptr()
+        a.transported = self.transported.data_ptr()
+        a.sc_p = self.sc_p.data_ptr()
+        a.nob = len(wl["block_end"])
+        a.last_accepted_dt = self.last_dt.data_ptr()
+        a.atol, a.rtol = self.atol.data_ptr(), self.rtol.data_ptr()
+        a.stream = torch.cuda.current_stream(dev).cuda_stream
+        self.args = a
+        self.ncell = g.ncell
+
+    def reset(self):
+        """Restore the initial state (for repeated timing of the same step)."""
+        self.sc_p.copy_(self.sc_p0)
+        self.last_dt.copy_(self.last_dt0)
+
+    def step(self):
+        rc = abi.check(abi.lib().brams_chem_rodas3_dyndt_device(C.byref(self.args)),
+                       "brams_chem_rodas3_dyndt_device")
+        return rc
+
+    def species_host(self):
+        """sc_p back on the host as a list of Fortran-ordered (m1,m2,m3) arrays."""
+        g = self.grid
+        arr = self.sc_p.cpu().numpy()
+        return [np.asfortranarray(arr[s].reshape((g.m1, g.m2, g.m3), order="F")) for s in range(arr.shape[0])]

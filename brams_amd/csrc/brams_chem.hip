@@ -1,0 +1,404 @@
+// C-ABI of the CCATT RODAS3 chemistry step on MI355X (include/brams_chem.h).
+//
+// Host side of the drop-in for chem_rodas3_dyndt
+// (src/brams/ccatt/chem_spack_rodas3_dyndt.f90:41-131): validates the
+// arguments, keeps persistent device buffers per grid size, moves host arrays
+// to HBM and back for the Fortran entry point, and launches the kernels of
+// rodas3_kernel.h:
+//   1. block 1 alone (its value-based sparsity mask changes every attempt and
+//      fixes the mask for every later block, rodas3:470-506);
+//   2. the remaining blocks, 128-lane workgroups for the leading blocks with
+//      more than 64 cells and wave64 workgroups for the rest.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/brams_chem.h"
+#include "rodas3_kernel.h"
+
+namespace {
+
+thread_local std::string g_err = "";
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(-100 - (int)e_, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// device buffer that only grows
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= n) return hipSuccess;
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) n = bytes;
+    return e;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct State {
+  bool init = false;
+  int device = 0;
+  // persistent per-call buffers
+  DevBuf block_start, mask, attempts, substeps, failed;
+  // host-entry staging
+  DevBuf sc_p, sc_t, sc_t_dyn, met, jphoto, last_dt, consts;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evb = nullptr, evm = nullptr;
+  long long main_cells = 0;
+  brams_chem_stats stats{};
+  std::vector<int> part_sizes;   // block sizes of the cached partition
+  long long part_ncell = -1;
+  std::vector<int> h_att, h_sub, h_fail;
+};
+State g;
+
+// AllocIndex, chem_spack_utils.f90:93-125
+int partition(int block_size, long long ncell, std::vector<int>& sizes) {
+  if (ncell <= 0 || block_size <= 0) return -1;
+  const int nob = (block_size >= ncell) ? 1 : (int)(ncell / block_size);
+  sizes.assign(nob, (block_size >= ncell) ? (int)ncell : block_size);
+  long long rest = ncell - (long long)sizes[0] * nob;
+  while (rest > 0)
+    for (int b = 0; b < nob && rest > 0; ++b) {
+      ++sizes[b];
+      --rest;
+    }
+  return nob;
+}
+
+int ensure_partition(long long ncell) {
+  if (g.part_ncell == ncell) return 0;
+  int nob = partition(64, ncell, g.part_sizes);
+  if (nob < 0) return fail(-2, "empty grid");
+  for (int s : g.part_sizes)
+    if (s > 128) return fail(-3, "block of %d cells exceeds 128", s);
+  std::vector<int> start(nob + 1, 0);
+  for (int b = 0; b < nob; ++b) start[b + 1] = start[b] + g.part_sizes[b];
+  HIP_TRY(g.block_start.ensure(sizeof(int) * (nob + 1)));
+  HIP_TRY(hipMemcpy(g.block_start.p, start.data(), sizeof(int) * (nob + 1), hipMemcpyHostToDevice));
+  HIP_TRY(g.attempts.ensure(sizeof(int) * nob));
+  HIP_TRY(g.substeps.ensure(sizeof(int) * nob));
+  HIP_TRY(g.failed.ensure(sizeof(int) * nob));
+  g.part_ncell = ncell;
+  return 0;
+}
+
+// Launch the blocks of one grid; p has every pointer filled except the
+// per-call bookkeeping arrays, which this sets.
+int launch_step(brams::StepParams p, hipStream_t st, int nob) {
+  p.block_start = g.block_start.as<int>();
+  p.mask = g.mask.as<unsigned>();
+  p.blk_attempts = g.attempts.as<int>();
+  p.blk_substeps = g.substeps.as<int>();
+  p.blk_failed = g.failed.as<int>();
+  const std::vector<int>& sz = g.part_sizes;
+  int nbig = 0;
+  while (nbig < nob && sz[nbig] > 64) ++nbig;
+  HIP_TRY(hipEventRecord(g.ev0, st));
+  if (sz[0] > 64)
+    hipLaunchKernelGGL((brams::rodas3_blocks<128, true>), dim3(1), dim3(128), 0, st, p, 0);
+  else
+    hipLaunchKernelGGL((brams::rodas3_blocks<64, true>), dim3(1), dim3(64), 0, st, p, 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(g.evb, st));
+  if (nbig > 1) {
+    hipLaunchKernelGGL((brams::rodas3_blocks<128, false>), dim3(nbig - 1), dim3(128), 0, st, p, 1);
+    HIP_TRY(hipGetLastError());
+  }
+  const int first64 = nbig > 1 ? nbig : 1;
+  HIP_TRY(hipEventRecord(g.evm, st));
+  g.main_cells = 0;
+  if (nob > first64) {
+    hipLaunchKernelGGL((brams::rodas3_blocks<64, false>), dim3(nob - first64), dim3(64), 0, st, p,
+                       first64);
+    HIP_TRY(hipGetLastError());
+    for (int b = first64; b < nob; ++b) g.main_cells += sz[b];
+  }
+  HIP_TRY(hipEventRecord(g.ev1, st));
+  return 0;
+}
+
+int collect_stats(hipStream_t st, int nob, long long ncell) {
+  HIP_TRY(hipStreamSynchronize(st));
+  g.h_att.resize(nob);
+  g.h_sub.resize(nob);
+  g.h_fail.resize(nob);
+  unsigned mw[MECH_MASK_WORDS];
+  HIP_TRY(hipMemcpy(g.h_att.data(), g.attempts.p, sizeof(int) * nob, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(g.h_sub.data(), g.substeps.p, sizeof(int) * nob, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(g.h_fail.data(), g.failed.p, sizeof(int) * nob, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(mw, g.mask.p, sizeof mw, hipMemcpyDeviceToHost));
+  brams_chem_stats s{};
+  s.cells = ncell;
+  s.blocks = nob;
+  for (int b = 0; b < nob; ++b) {
+    s.attempts += g.h_att[b];
+    s.substeps += g.h_sub[b];
+    s.failed_blocks += g.h_fail[b];
+    s.cell_attempts += (long long)g.h_att[b] * g.part_sizes[b];
+    s.cell_substeps += (long long)g.h_sub[b] * g.part_sizes[b];
+  }
+  int nz = 0;
+  for (int w = 0; w < MECH_MASK_WORDS; ++w) nz += __builtin_popcount(mw[w]);
+  s.mask_nnz = nz;
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+  s.kernel_ms = ms;
+  HIP_TRY(hipEventElapsedTime(&ms, g.ev0, g.evb));
+  s.block1_ms = ms;
+  HIP_TRY(hipEventElapsedTime(&ms, g.evm, g.ev1));
+  s.main_ms = ms;
+  s.main_cells = g.main_cells;
+  g.stats = s;
+  return (int)s.failed_blocks;
+}
+
+int check_common(int m1, int m2, int m3, int ia, int iz, int ja, int jz, int ndyn, int split,
+                 int photoj, float dtlt) {
+  if (m1 < 3 || m2 < 1 || m3 < 1) return fail(-2, "bad grid extents %d %d %d", m1, m2, m3);
+  if (ia < 1 || iz > m2 || ia > iz || ja < 1 || jz > m3 || ja > jz)
+    return fail(-2, "bad interior bounds ia=%d iz=%d ja=%d jz=%d", ia, iz, ja, jz);
+  if (ndyn < 1) return fail(-2, "N_DYN_CHEM must be >= 1");
+  if (split < 0 || split > 2) return fail(-2, "unknown split_method %d", split);
+  if (photoj == BRAMS_PHOTOJ_LUT)
+    return fail(-4, "PhotojMethod LUT is not supported (RELACS_TUV uses FAST-TUV)");
+  if (photoj < 0 || photoj > 2) return fail(-2, "unknown PhotojMethod %d", photoj);
+  if (!(dtlt > 0.f)) return fail(-2, "dtlt must be positive");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int brams_chem_init(int device) {
+  if (g.init && g.device == device) return 0;
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(-1, "device %d not present (%d devices)", device, n);
+  HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(-1, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+  g.device = device;
+  if (!g.ev0) HIP_TRY(hipEventCreate(&g.ev0));
+  if (!g.ev1) HIP_TRY(hipEventCreate(&g.ev1));
+  if (!g.evb) HIP_TRY(hipEventCreate(&g.evb));
+  if (!g.evm) HIP_TRY(hipEventCreate(&g.evm));
+  HIP_TRY(g.mask.ensure(sizeof(unsigned) * MECH_MASK_WORDS));
+  g.init = true;
+  return 0;
+}
+
+int brams_chem_alloc_index(int32_t block_size, int32_t m1, int32_t ia, int32_t iz, int32_t ja,
+                           int32_t jz, int32_t* block_end, int32_t max_nob) {
+  std::vector<int> sz;
+  long long ncell = (long long)(m1 - 2) * (iz - ia + 1) * (jz - ja + 1);
+  int nob = partition(block_size, ncell, sz);
+  if (nob < 0) return fail(-2, "empty grid");
+  if (block_end) {
+    if (nob > max_nob) return fail(-2, "max_nob %d < nob %d", max_nob, nob);
+    for (int b = 0; b < nob; ++b) block_end[b] = sz[b];
+  }
+  return nob;
+}
+
+int brams_chem_rodas3_dyndt_device(const brams_chem_dev_args* a) {
+  if (!a) return fail(-2, "null args");
+  if (!g.init) {
+    int rc = brams_chem_init(0);
+    if (rc) return rc;
+  }
+  int rc = check_common(a->m1, a->m2, a->m3, a->ia, a->iz, a->ja, a->jz, a->n_dyn_chem,
+                        a->split_method, a->photoj_method, a->dtlt);
+  if (rc) return rc;
+  if (!a->pp || !a->pi0 || !a->theta || !a->rv || !a->jphoto || !a->weight || !a->transported ||
+      !a->sc_p || !a->last_accepted_dt || !a->atol || !a->rtol)
+    return fail(-2, "null device pointer");
+  if (a->split_method == BRAMS_SPLIT_PARALLEL && a->n_dyn_chem == 1 && !a->sc_t)
+    return fail(-2, "PARALLEL with N_DYN_CHEM=1 needs sc_t");
+  if (a->split_method == BRAMS_SPLIT_PARALLEL && a->n_dyn_chem > 1 && !a->sc_t_dyn)
+    return fail(-2, "PARALLEL with N_DYN_CHEM>1 needs sc_t_dyn");
+  const long long ncell = (long long)(a->m1 - 2) * (a->iz - a->ia + 1) * (a->jz - a->ja + 1);
+  if ((rc = ensure_partition(ncell))) return rc;
+  const int nob = (int)g.part_sizes.size();
+  if (a->nob != nob) return fail(-2, "nob %d does not match the AllocIndex partition (%d)", a->nob, nob);
+  brams::StepParams p{};
+  p.m1 = a->m1; p.m2 = a->m2; p.m3 = a->m3;
+  p.ia = a->ia; p.iz = a->iz; p.ja = a->ja; p.jz = a->jz;
+  p.nk = a->m1 - 2; p.ni = a->iz - a->ia + 1;
+  p.dtlt = a->dtlt; p.ndyn = a->n_dyn_chem; p.split = a->split_method;
+  p.cp = a->cp; p.cpor = a->cpor; p.p00 = a->p00;
+  p.pp = a->pp; p.pi0 = a->pi0; p.theta = a->theta; p.rv = a->rv;
+  p.jphoto = a->jphoto;
+  p.weight = a->weight; p.transported = a->transported;
+  p.sc_p = a->sc_p; p.sc_t = a->sc_t; p.sc_t_dyn = a->sc_t_dyn;
+  p.n3d = (long long)a->m1 * a->m2 * a->m3;
+  p.last_dt = a->last_accepted_dt;
+  p.atol = a->atol; p.rtol = a->rtol;
+  hipStream_t st = (hipStream_t)a->stream;
+  if ((rc = launch_step(p, st, nob))) return rc;
+  return collect_stats(st, nob, ncell);
+}
+
+int brams_chem_rodas3_dyndt(const brams_chem_args* a) {
+  if (!a) return fail(-2, "null args");
+  if (!g.init) {
+    int rc = brams_chem_init(0);
+    if (rc) return rc;
+  }
+  int rc = check_common(a->m1, a->m2, a->m3, a->ia, a->iz, a->ja, a->jz, a->n_dyn_chem,
+                        a->split_method, a->photoj_method, a->dtlt);
+  if (rc) return rc;
+  if (a->nspecies != MECH_NS || a->nr != MECH_NR || a->nr_photo != MECH_NPHOT)
+    return fail(-2, "mechanism sizes %d/%d/%d differ from RELACS_TUV %d/%d/%d", a->nspecies, a->nr,
+                a->nr_photo, MECH_NS, MECH_NR, MECH_NPHOT);
+  const long long ncell = (long long)(a->m1 - 2) * (a->iz - a->ia + 1) * (a->jz - a->ja + 1);
+  if ((rc = ensure_partition(ncell))) return rc;
+  const int nob = (int)g.part_sizes.size();
+  if (a->nob != nob) return fail(-2, "nob %d does not match the AllocIndex partition (%d)", a->nob, nob);
+  for (int b = 0; b < nob; ++b)
+    if (a->block_end[b] != g.part_sizes[b])
+      return fail(-2, "block_end(%d)=%d differs from AllocIndex (%d)", b + 1, a->block_end[b],
+                  g.part_sizes[b]);
+  // transport flags from the caller's index lists
+  int flags[MECH_NS];
+  for (int s = 0; s < MECH_NS; ++s) flags[s] = -1;
+  for (int t = 0; t < a->n_transported; ++t) {
+    int n = a->transp_chem_index[t];
+    if (n < 1 || n > MECH_NS) return fail(-2, "transp_chem_index out of range");
+    flags[n - 1] = 1;
+  }
+  for (int t = 0; t < a->n_not_transported; ++t) {
+    int n = a->no_transp_chem_index[t];
+    if (n < 1 || n > MECH_NS) return fail(-2, "no_transp_chem_index out of range");
+    flags[n - 1] = 0;
+  }
+  for (int s = 0; s < MECH_NS; ++s)
+    if (flags[s] < 0) return fail(-2, "species %d in neither transport list", s + 1);
+  const bool need_t = a->split_method == BRAMS_SPLIT_PARALLEL && a->n_dyn_chem == 1;
+  const bool need_td = a->split_method == BRAMS_SPLIT_PARALLEL && a->n_dyn_chem > 1;
+  if (need_t && !a->sc_t) return fail(-2, "PARALLEL with N_DYN_CHEM=1 needs sc_t");
+  if (need_td && !a->sc_t_dyn) return fail(-2, "PARALLEL with N_DYN_CHEM>1 needs sc_t_dyn");
+
+  const long long n3d = (long long)a->m1 * a->m2 * a->m3;
+  const size_t fsz = sizeof(float) * n3d;
+  HIP_TRY(g.sc_p.ensure(fsz * MECH_NS));
+  HIP_TRY(g.met.ensure(fsz * 4));
+  HIP_TRY(g.jphoto.ensure(sizeof(double) * MECH_NPHOT * n3d));
+  HIP_TRY(g.last_dt.ensure(sizeof(double) * nob));
+  // consts: weight[47] f32 | transported[47] i32 | atol[47] f64 | rtol[47] f64
+  const size_t coff_w = 0, coff_t = 256, coff_a = 512, coff_r = 1024;
+  HIP_TRY(g.consts.ensure(1536));
+  if (need_t) HIP_TRY(g.sc_t.ensure(fsz * MECH_NS));
+  if (need_td) HIP_TRY(g.sc_t_dyn.ensure(fsz * MECH_NS));
+  hipStream_t st = nullptr;
+  char* sc = g.sc_p.as<char>();
+  for (int s = 0; s < MECH_NS; ++s)
+    HIP_TRY(hipMemcpyAsync(sc + s * fsz, a->sc_p[s], fsz, hipMemcpyHostToDevice, st));
+  if (need_t)
+    for (int s = 0; s < MECH_NS; ++s)
+      HIP_TRY(hipMemcpyAsync(g.sc_t.as<char>() + s * fsz, a->sc_t[s], fsz, hipMemcpyHostToDevice, st));
+  if (need_td)
+    for (int s = 0; s < MECH_NS; ++s)
+      HIP_TRY(hipMemcpyAsync(g.sc_t_dyn.as<char>() + s * fsz, a->sc_t_dyn[s], fsz,
+                             hipMemcpyHostToDevice, st));
+  char* met = g.met.as<char>();
+  HIP_TRY(hipMemcpyAsync(met + 0 * fsz, a->pp, fsz, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(met + 1 * fsz, a->pi0, fsz, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(met + 2 * fsz, a->theta, fsz, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(met + 3 * fsz, a->rv, fsz, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(g.jphoto.p, a->jphoto, sizeof(double) * MECH_NPHOT * n3d,
+                         hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(g.last_dt.p, a->last_accepted_dt, sizeof(double) * nob,
+                         hipMemcpyHostToDevice, st));
+  char cbuf[1536];
+  std::memcpy(cbuf + coff_w, a->weight, sizeof(float) * MECH_NS);
+  std::memcpy(cbuf + coff_t, flags, sizeof(int) * MECH_NS);
+  std::memcpy(cbuf + coff_a, a->atol, sizeof(double) * MECH_NS);
+  std::memcpy(cbuf + coff_r, a->rtol, sizeof(double) * MECH_NS);
+  HIP_TRY(hipMemcpy(g.consts.p, cbuf, sizeof cbuf, hipMemcpyHostToDevice));
+
+  brams::StepParams p{};
+  p.m1 = a->m1; p.m2 = a->m2; p.m3 = a->m3;
+  p.ia = a->ia; p.iz = a->iz; p.ja = a->ja; p.jz = a->jz;
+  p.nk = a->m1 - 2; p.ni = a->iz - a->ia + 1;
+  p.dtlt = a->dtlt; p.ndyn = a->n_dyn_chem; p.split = a->split_method;
+  p.cp = a->cp; p.cpor = a->cpor; p.p00 = a->p00;
+  p.pp = (const float*)(met + 0 * fsz);
+  p.pi0 = (const float*)(met + 1 * fsz);
+  p.theta = (const float*)(met + 2 * fsz);
+  p.rv = (const float*)(met + 3 * fsz);
+  p.jphoto = g.jphoto.as<double>();
+  p.weight = (const float*)(g.consts.as<char>() + coff_w);
+  p.transported = (const int*)(g.consts.as<char>() + coff_t);
+  p.atol = (const double*)(g.consts.as<char>() + coff_a);
+  p.rtol = (const double*)(g.consts.as<char>() + coff_r);
+  p.sc_p = g.sc_p.as<float>();
+  p.sc_t = need_t ? g.sc_t.as<float>() : nullptr;
+  p.sc_t_dyn = need_td ? g.sc_t_dyn.as<float>() : nullptr;
+  p.n3d = n3d;
+  p.last_dt = g.last_dt.as<double>();
+  if ((rc = launch_step(p, st, nob))) return rc;
+  // results back into the caller's arrays
+  if (need_t) {
+    for (int s = 0; s < MECH_NS; ++s)
+      HIP_TRY(hipMemcpyAsync(a->sc_t[s], g.sc_t.as<char>() + s * fsz, fsz, hipMemcpyDeviceToHost, st));
+  } else {
+    for (int s = 0; s < MECH_NS; ++s)
+      HIP_TRY(hipMemcpyAsync(a->sc_p[s], sc + s * fsz, fsz, hipMemcpyDeviceToHost, st));
+  }
+  HIP_TRY(hipMemcpyAsync(a->last_accepted_dt, g.last_dt.p, sizeof(double) * nob,
+                         hipMemcpyDeviceToHost, st));
+  return collect_stats(st, nob, ncell);
+}
+
+int brams_chem_get_stats(brams_chem_stats* out) {
+  if (!out) return fail(-2, "null stats");
+  *out = g.stats;
+  return 0;
+}
+
+const char* brams_chem_last_error(void) { return g_err.c_str(); }
+
+void brams_chem_finalize(void) {
+  DevBuf* bufs[] = {&g.block_start, &g.mask, &g.attempts, &g.substeps, &g.failed, &g.sc_p,
+                    &g.sc_t, &g.sc_t_dyn, &g.met, &g.jphoto, &g.last_dt, &g.consts};
+  for (DevBuf* b : bufs) b->release();
+  hipEvent_t* evs[] = {&g.ev0, &g.ev1, &g.evb, &g.evm};
+  for (hipEvent_t* e : evs) {
+    if (*e) hipEventDestroy(*e);
+    *e = nullptr;
+  }
+  g.part_ncell = -1;
+  g.init = false;
+}
+
+}  // extern "C"

@@ -1,0 +1,290 @@
+// Batched FP64 RODAS3 integration of the RELACS_TUV mechanism on CDNA4.
+//
+// Mapping: one workgroup = one reference block (chem_spack_utils.f90:93-160),
+// one lane = one cell.  Blocks of 64 cells are exactly one wave64; blocks of
+// 65..127 cells run as a 128-lane workgroup.  The block-shared step size of
+// the reference (rodas3:634-700) is a workgroup reduction of the per-cell
+// error, so every lane takes the same accept/reject branch.
+//
+// State never leaves the lane: gather from the Fortran (k,i,j) arrays, rate
+// coefficients, the sub-step loop (Jacobian, masked LU, four stages, error)
+// and the write-back all happen in one kernel.  The Jacobian is not stored:
+// each attempt rebuilds 1/(gamma h) I - J from y0 and rk inside the
+// factorisation, which yields the same values the reference keeps in DLdrdc
+// (rejections recompute neither J nor f(y0), rodas3:424-457).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "generated/relacs_tuv_gen.h"
+
+namespace brams {
+
+struct StepParams {
+  int m1, m2, m3, ia, iz, ja, jz;
+  int nk, ni;                    // interior levels (m1-2) and columns in i
+  float dtlt;
+  int ndyn, split;
+  float cp, cpor, p00;
+  const float *pp, *pi0, *theta, *rv;
+  const double* jphoto;          // (17, m1, m2, m3)
+  const float* weight;           // [47]
+  const int* transported;        // [47] 1/0
+  float* sc_p;                   // [47][n3d]
+  float* sc_t;                   // [47][n3d] or null
+  const float* sc_t_dyn;         // [47][n3d] or null
+  long long n3d;
+  const int* block_start;        // [nob+1]
+  double* last_dt;               // [nob]
+  const double *atol, *rtol;     // [47]
+  unsigned* mask;                // [MECH_MASK_WORDS]: block-1 mask, read by later blocks
+  int* blk_attempts;             // [nob]
+  int* blk_substeps;             // [nob]
+  int* blk_failed;               // [nob]
+};
+
+enum { kSplitSymmetric = 0, kSplitSequential = 1, kSplitParallel = 2 };
+// Upper bound on attempts per block and call.  Each rejection shrinks dt by
+// at least 10% until dt_min forces acceptance (rodas3:650-676), so a block
+// needs far fewer (at most dtlt/dt_min accepted steps plus a few dozen
+// rejections each); the bound only guards against a non-terminating wave.
+constexpr int kMaxAttempts = 20000;
+
+template <int WG>
+__device__ __forceinline__ void block_reduce(double& errmax, int& rej, int& fail,
+                                             double* s_err, int* s_rej, int* s_fail) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    errmax = fmax(errmax, __shfl_xor(errmax, off, 64));
+    rej |= __shfl_xor(rej, off, 64);
+    fail |= __shfl_xor(fail, off, 64);
+  }
+  if constexpr (WG > 64) {
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+      s_err[wave] = errmax;
+      s_rej[wave] = rej;
+      s_fail[wave] = fail;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < WG / 64; ++w) {
+      errmax = fmax(errmax, s_err[w]);
+      rej |= s_rej[w];
+      fail |= s_fail[w];
+    }
+    __syncthreads();
+  }
+}
+
+// DYN: block 1 of the call.  Its sparsity mask is recomputed from cell 1 on
+// every attempt (rodas3:470-482) and the last one is published for the other
+// blocks.  Otherwise the mask is read from p.mask.
+template <int WG, bool DYN>
+__global__ __launch_bounds__(WG) void rodas3_blocks(StepParams p, int block0) {
+  const int b = block0 + blockIdx.x;
+  const int start = p.block_start[b];
+  const int ncell = p.block_start[b + 1] - start;
+  const int c = threadIdx.x;
+  const bool act = c < ncell;
+  const long long g = start + (act ? c : 0);        // idle lanes shadow cell 1
+
+  // cell position, j -> i -> k order (chem_spack_utils.f90:144-160)
+  const long long kk = 2 + g % p.nk;
+  const long long t = g / p.nk;
+  const long long ii = p.ia + t % p.ni;
+  const long long jj = p.ja + t / p.ni;
+  const long long o = (kk - 1) + (long long)p.m1 * (ii - 1) + (long long)p.m1 * p.m2 * (jj - 1);
+
+  // ---- gather and unit conversion, rodas3:226-354
+  const float exner = (p.pp[o] + p.pi0[o]) / p.cp;   // REAL arithmetic
+  double press = (double)exner;
+  const double temp = (double)p.theta[o] * press;
+  press = pow(press, (double)p.cpor) * (double)p.p00;
+  const double vapp = (double)p.rv[o];
+  const double volmol = (6.02e23 * 1e-15 * 28.96) * press / (8.314 * temp);
+  const double volmol_i = 1.0 / volmol;
+  const bool par_n = (p.split == kSplitParallel) && (p.ndyn > 1);
+
+  double y[MECH_NS];
+#pragma unroll
+  for (int s = 0; s < MECH_NS; ++s) {
+    const float v = p.sc_p[(long long)s * p.n3d + o];
+    if (!p.transported[s]) {
+      y[s] = (double)v;
+    } else if (par_n) {
+      const float st = v - p.sc_t_dyn[(long long)s * p.n3d + o] * (float)p.ndyn * p.dtlt;
+      y[s] = (double)st * volmol / (double)p.weight[s];
+    } else {
+      const double x = (double)v * volmol / (double)p.weight[s];
+      y[s] = x > 0.0 ? x : 0.0;
+    }
+  }
+  double rk[MECH_NR];
+  {
+    double J[MECH_NPHOT];
+#pragma unroll
+    for (int n = 0; n < MECH_NPHOT; ++n) J[n] = p.jphoto[n + (long long)MECH_NPHOT * o];
+    mech_kinetic(temp, press, vapp, J, rk);
+  }
+
+  // ---- RODAS3 sub-steps, rodas3:413-705
+  const double c83 = 8.0 / 3.0;
+  const float tnf = p.dtlt * (float)p.ndyn;
+  const double dt_min = fmax(1.0, 1.0e-2 * (double)tnf);
+  const double dt_max = (double)tnf;
+  const double time_f = (double)tnf;
+  double dt = p.last_dt[b];
+  double last_dt = dt;
+  double time_c = 0.0;
+  int natt = 0, nsub = 0, failed = 0;
+
+  __shared__ unsigned s_mask[MECH_MASK_WORDS];
+  __shared__ double s_err[WG / 64 + 1];
+  __shared__ int s_rej[WG / 64 + 1], s_fail[WG / 64 + 1];
+  unsigned mw[MECH_MASK_WORDS];
+  if constexpr (!DYN) {
+#pragma unroll
+    for (int w = 0; w < MECH_MASK_WORDS; ++w) mw[w] = p.mask[w];
+  }
+
+  while (time_c + 1.0e-8 < time_f) {
+    double f0[MECH_NS];
+    mech_fun(rk, y, f0);
+    for (;;) {
+      ++natt;
+      const double igdt = 1.0 / (0.5 * dt);
+      if constexpr (DYN) {
+        if (c == 0) {
+          unsigned m[MECH_MASK_WORDS];
+          mech_mask(rk, y, igdt, m);
+#pragma unroll
+          for (int w = 0; w < MECH_MASK_WORDS; ++w) s_mask[w] = m[w];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < MECH_MASK_WORDS; ++w) mw[w] = s_mask[w];
+      }
+      double lu[MECH_NLU];
+      const int bad = mech_factor(rk, y, igdt, mw, lu);
+      const double dti = 1.0 / dt;
+      // stage 1, rodas3:523-527
+      double k1[MECH_NS];
+#pragma unroll
+      for (int s = 0; s < MECH_NS; ++s) k1[s] = f0[s];
+      mech_solve(lu, k1);
+      // stage 2, rodas3:534-538
+      double k2[MECH_NS];
+#pragma unroll
+      for (int s = 0; s < MECH_NS; ++s) k2[s] = (4.0 * dti) * k1[s] + f0[s];
+      mech_solve(lu, k2);
+      // stage 3 with clipping, rodas3:542-566
+      double yn[MECH_NS], ft[MECH_NS];
+#pragma unroll
+      for (int s = 0; s < MECH_NS; ++s) {
+        yn[s] = y[s] + 2.0 * k1[s];
+        if (yn[s] < 0.0) {
+          yn[s] = 0.0;
+          k1[s] = 0.5 * (yn[s] - y[s]);
+        }
+      }
+      mech_fun(rk, yn, ft);
+      double k3[MECH_NS];
+#pragma unroll
+      for (int s = 0; s < MECH_NS; ++s) {
+        k2[s] = k1[s] - k2[s];                    // k1 - k2, reused below
+        k3[s] = ft[s] + dti * k2[s];
+      }
+      mech_solve(lu, k3);
+      // stage 4 with clipping, rodas3:573-599
+#pragma unroll
+      for (int s = 0; s < MECH_NS; ++s) {
+        yn[s] = y[s] + 2.0 * k1[s] + k3[s];
+        if (yn[s] < 0.0) {
+          yn[s] = 0.0;
+          k3[s] = (yn[s] - y[s]) - 2.0 * k1[s];
+        }
+      }
+      mech_fun(rk, yn, ft);
+#pragma unroll
+      for (int s = 0; s < MECH_NS; ++s) ft[s] = ft[s] + dti * (k2[s] - c83 * k3[s]);
+      mech_solve(lu, ft);                          // ft now holds k4
+      // solution and error, rodas3:609-648
+      double e = 0.0;
+#pragma unroll
+      for (int s = 0; s < MECH_NS; ++s) {
+        double v = y[s] + 2.0 * k1[s] + k3[s] + ft[s];
+        v = fmax(v, 0.0);
+        yn[s] = v;
+        const double tol = p.atol[s] + p.rtol[s] * fmax(fabs(y[s]), fabs(v));
+        const double q = ft[s] / tol;
+        e = e + q * q;
+      }
+      const double err = fmax(1.0e-15, sqrt(e / (double)MECH_NS));
+      double errmax = act ? err : 0.0;
+      int rej = (act && (err - 1.0e-8 > 1.0)) ? 1 : 0;
+      // a zero pivot or a non-finite error sum ends the block (the reference
+      // aborts the process there); kMaxAttempts bounds every wave's work
+      int fail = (act && (bad || !isfinite(e))) ? 1 : 0;
+      if (natt >= kMaxAttempts) fail = 1;
+      block_reduce<WG>(errmax, rej, fail, s_err, s_rej, s_fail);
+      // controller, rodas3:650-700
+      const double fac = fmin(6.0, fmax(0.2, 0.9 / pow(errmax, 1.0 / 3.0)));
+      double dt_new = dt * fac;
+      dt_new = fmax(dt_min, fmin(dt_max, dt_new));
+      if (fail) {
+        failed = 1;
+        time_c = time_f;
+        break;
+      }
+      if (rej && dt_new > dt_min) {
+        dt = dt_new;
+        continue;
+      }
+      time_c = time_c + dt;
+      dt = fmin(dt_new, time_f - time_c);
+      if (time_c < time_f) last_dt = dt_new;
+#pragma unroll
+      for (int s = 0; s < MECH_NS; ++s) y[s] = yn[s];
+      ++nsub;
+      break;
+    }
+  }
+
+  // ---- write-back, rodas3:713-798
+  if (act) {
+#pragma unroll
+    for (int s = 0; s < MECH_NS; ++s) {
+      const long long idx = (long long)s * p.n3d + o;
+      if (!p.transported[s]) {
+        const float v = (float)y[s];
+        p.sc_p[idx] = v > 0.0f ? v : 0.0f;
+        continue;
+      }
+      const double conv = y[s] * (double)p.weight[s] * volmol_i;
+      if (par_n) {
+        const float base = p.sc_t_dyn[idx] * (float)p.ndyn * p.dtlt;
+        const float v = (float)((double)base + conv);
+        p.sc_p[idx] = v > 0.0f ? v : 0.0f;
+      } else if (p.split == kSplitParallel) {
+        const double dti = 1.0 / (double)p.dtlt;
+        p.sc_t[idx] = (float)((double)p.sc_t[idx] + (conv - (double)p.sc_p[idx]) * dti);
+      } else {
+        const float v = (float)conv;
+        p.sc_p[idx] = v > 0.0f ? v : 0.0f;
+      }
+    }
+  }
+  if (c == 0) {
+    p.last_dt[b] = last_dt;
+    p.blk_attempts[b] = natt;
+    p.blk_substeps[b] = nsub;
+    p.blk_failed[b] = failed;
+    if constexpr (DYN) {
+#pragma unroll
+      for (int w = 0; w < MECH_MASK_WORDS; ++w) p.mask[w] = s_mask[w];
+    }
+  }
+}
+
+}  // namespace brams

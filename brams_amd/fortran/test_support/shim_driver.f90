@@ -1,0 +1,57 @@
+! Test driver of the Fortran shim: reads one chemistry call from a stream
+! file written by tests/test_gpu_fortran_shim.py, calls chem_rodas3_dyndt
+! exactly as chemistry.f90:443-453 does, and writes the updated state.
+PROGRAM shim_driver
+  USE mem_chem1, ONLY: chem1_vars
+  USE mem_spack, ONLY: spack_type, spack_type_2d
+  USE Extras, ONLY: ext3d
+  USE mod_chem_spack_rodas3_dyndt, ONLY: chem_rodas3_dyndt
+  IMPLICIT NONE
+  INTEGER :: m1, m2, m3, nob, mbs, nsp, nr, nrp, ntr, nnt, ndyn, n, u
+  REAL :: dtlt, cp, cpor, p00
+  REAL, ALLOCATABLE :: pp(:,:,:), pi0(:,:,:), theta(:,:,:), rv(:,:,:), dn0(:,:,:), rcp(:,:,:)
+  REAL, ALLOCATABLE :: cosz(:,:), att(:,:), weight(:)
+  DOUBLE PRECISION, ALLOCATABLE :: jphoto(:,:,:,:), last_dt(:), atol(:), rtol(:)
+  INTEGER, ALLOCATABLE :: tidx(:), nidx(:), block_end(:), ik(:,:), ii(:,:), ij(:,:), kij(:,:)
+  TYPE(chem1_vars), ALLOCATABLE :: chem1_g(:)
+  TYPE(spack_type) :: spack(1)
+  TYPE(spack_type_2d) :: spack_2d(1,1)
+  TYPE(ext3d) :: extra3d(1)
+  LOGICAL :: gnz
+  CHARACTER(LEN=256) :: fin, fout
+  CHARACTER(LEN=10) :: photoj
+  CHARACTER(LEN=20) :: split
+
+  CALL get_command_argument(1, fin)
+  CALL get_command_argument(2, fout)
+  OPEN(newunit=u, file=TRIM(fin), access='stream', form='unformatted', status='old')
+  READ(u) m1, m2, m3, nob, mbs, nsp, nr, nrp, ntr, nnt, ndyn
+  READ(u) dtlt, cp, cpor, p00
+  ALLOCATE(pp(m1,m2,m3), pi0(m1,m2,m3), theta(m1,m2,m3), rv(m1,m2,m3))
+  ALLOCATE(dn0(m1,m2,m3), rcp(m1,m2,m3), cosz(m2,m3), att(m2,m3))
+  ALLOCATE(weight(nsp), jphoto(nrp,m1,m2,m3), last_dt(nob), atol(nsp), rtol(nsp))
+  ALLOCATE(tidx(200), nidx(200), block_end(nob), ik(mbs,nob), ii(mbs,nob), ij(mbs,nob), kij(mbs,nob))
+  tidx = 0; nidx = 0; dn0 = 1.0; rcp = 0.0; cosz = 0.0; att = 1.0
+  READ(u) pp, pi0, theta, rv, weight, jphoto, last_dt, atol, rtol
+  READ(u) tidx(1:ntr), nidx(1:nnt), block_end, ik, ii, ij
+  kij = 0
+  ALLOCATE(chem1_g(nsp))
+  DO n = 1, nsp
+     ALLOCATE(chem1_g(n)%sc_p(m1,m2,m3))
+     READ(u) chem1_g(n)%sc_p
+  END DO
+  CLOSE(u)
+  photoj = 'FAST-TUV'
+  split = 'SYMMETRIC'
+  gnz = .FALSE.
+  CALL chem_rodas3_dyndt(m1,m2,m3,dtlt,pp,pi0,theta,rv,dn0,cosz,rcp, &
+                         nsp,nr,nrp,weight,photoj,200,ntr,nnt,tidx,nidx,chem1_g,nob,mbs, &
+                         block_end,ik,ii,ij,kij,last_dt,spack,spack_2d,atol,rtol,gnz, &
+                         ndyn,split,cp,cpor,p00,1,extra3d,jphoto,att)
+  OPEN(newunit=u, file=TRIM(fout), access='stream', form='unformatted', status='replace')
+  DO n = 1, nsp
+     WRITE(u) chem1_g(n)%sc_p
+  END DO
+  WRITE(u) last_dt
+  CLOSE(u)
+END PROGRAM shim_driver

@@ -73,7 +73,9 @@ typedef struct brams_chem_args {
 /*
  * The same step with every array already resident in device memory (HBM).
  * Species state is one contiguous buffer [nspecies][m1*m2*m3] (species-major);
- * sc_t / sc_t_dyn likewise.  last_accepted_dt and block_end live on device.
+ * sc_t / sc_t_dyn likewise.  The block partition is the AllocIndex one with
+ * block size 64 (the reference's block_size_ros, chemistry.f90:275); the
+ * library derives it from the grid.  last_accepted_dt lives on device.
  */
 typedef struct brams_chem_dev_args {
     int32_t m1, m2, m3, ia, iz, ja, jz;
@@ -87,8 +89,7 @@ typedef struct brams_chem_dev_args {
     float *sc_p;                 /* device [nspecies][m1*m2*m3] */
     float *sc_t;                 /* device or NULL */
     const float *sc_t_dyn;       /* device or NULL */
-    int32_t nob;
-    const int32_t *block_start;  /* device (nob+1) prefix sums of block sizes */
+    int32_t nob;                 /* must equal brams_chem_alloc_index(64, ...) of this grid */
     double *last_accepted_dt;    /* device (nob) */
     const double *atol, *rtol;   /* device (nspecies) */
     void *stream;                /* hipStream_t, NULL = default stream */
@@ -103,6 +104,11 @@ typedef struct brams_chem_stats {
     int64_t failed_blocks;       /* blocks with a zero pivot or non-finite state */
     int32_t mask_nnz;            /* entries kept by the value-based sparsity mask */
     float kernel_ms;             /* device time of the last step, all kernels */
+    int64_t cell_substeps;       /* sum over cells of their block's accepted sub-steps */
+    int64_t cell_attempts;       /* sum over cells of their block's attempts */
+    float block1_ms;             /* device time of the block-1 kernel (mask, serial prefix) */
+    float main_ms;               /* device time of the wave64 kernel over the 64-cell blocks */
+    int64_t main_cells;          /* cells that kernel integrated */
 } brams_chem_stats;
 
 /* Select the HIP device and load the mechanism tables.  Idempotent. */

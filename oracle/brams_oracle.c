@@ -463,11 +463,174 @@ static void set_mask_from_cell0(drv_t *D, const double *jac0, double igdt)
         }
 }
 
+/* ------------------------------------------------------------------ RODAS3
+ * The four stages of one attempt for one cell, rodas3:523-599 (the Rodas3
+ * tableau of Sandu et al. 1997 with gamma = 1/2), with the reference's
+ * non-negativity clipping of the stage states.  f and the linear solve with
+ * the factored 1/(gamma h) I - J come in as callbacks, so the same code runs
+ * the chemistry and the linear test problem of ora_rodas3_linear(). */
+typedef void (*stage_fun_fn)(void *ctx, const double *y, double *f);
+typedef void (*stage_solve_fn)(void *ctx, const double *rhs, double *x);
+
+static void rodas3_stages(int n, void *ctx, stage_fun_fn fun, stage_solve_fn solve,
+                          const double *y0, const double *f0, double dt, double *k1, double *k2,
+                          double *k3, double *k4, double *yn)
+{
+    const double c83 = 8.0 / 3.0;
+    const double dti = 1.0 / dt;
+    double bb[NS], ft[NS];
+    /* stage 1, rodas3:523-527 */
+    solve(ctx, f0, k1);
+    /* stage 2, rodas3:534-538 */
+    for (int s = 0; s < n; s++) bb[s] = (4.0 * dti) * k1[s] + f0[s];
+    solve(ctx, bb, k2);
+    /* stage 3 with clipping, rodas3:542-566 */
+    for (int s = 0; s < n; s++) {
+        yn[s] = y0[s] + 2.0 * k1[s];
+        if (yn[s] < 0.0) { yn[s] = 0.0; k1[s] = 0.5 * (yn[s] - y0[s]); }
+    }
+    fun(ctx, yn, ft);
+    for (int s = 0; s < n; s++) bb[s] = ft[s] + dti * (k1[s] - k2[s]);
+    solve(ctx, bb, k3);
+    /* stage 4 with clipping, rodas3:573-599 */
+    for (int s = 0; s < n; s++) {
+        yn[s] = y0[s] + 2.0 * k1[s] + k3[s];
+        if (yn[s] < 0.0) { yn[s] = 0.0; k3[s] = (yn[s] - y0[s]) - 2.0 * k1[s]; }
+    }
+    fun(ctx, yn, ft);
+    for (int s = 0; s < n; s++) bb[s] = ft[s] + dti * (k1[s] - k2[s] - c83 * k3[s]);
+    solve(ctx, bb, k4);
+}
+
+/* Solution y1 = max(y0 + 2 k1 + k3 + k4, 0) and error norm (k4 is the error
+ * estimate), rodas3:609-648.  Returns the raw sum of squares. */
+static double rodas3_solution(int n, const double *y0, const double *k1, const double *k3,
+                              const double *k4, const double *atol, const double *rtol,
+                              double *y1, double *err)
+{
+    double e = 0.0;
+    for (int s = 0; s < n; s++) {
+        double v = y0[s] + 2.0 * k1[s] + k3[s] + k4[s];
+        v = fmax(v, 0.0);
+        y1[s] = v;
+        double tol = atol[s] + rtol[s] * fmax(fabs(y0[s]), fabs(v));
+        double q = k4[s] / tol;
+        e = e + q * q;
+    }
+    *err = fmax(1.0e-15, sqrt(e / n));
+    return e;
+}
+
+/* New step from the block's largest error, rodas3:654-662:
+ * dt * min(6, max(0.2, 0.9 err^-1/3)) clamped to [dt_min, dt_max]. */
+static double rodas3_dt_new(double dt, double max_err, double dt_min, double dt_max)
+{
+    double fac = fmin(6.0, fmax(0.2, 0.9 / pow(max_err, 1.0 / 3.0)));
+    double dt_new = dt * fac;
+    return fmax(dt_min, fmin(dt_max, dt_new));
+}
+
+double ora_dt_new(double dt, double max_err, double dt_min, double dt_max)
+{
+    return rodas3_dt_new(dt, max_err, dt_min, dt_max);
+}
+
+typedef struct { drv_t *D; lumat *lu; const double *rk; } stage_ctx_t;
+
+static void stage_fun(void *vctx, const double *y, double *f)
+{
+    stage_ctx_t *ctx = (stage_ctx_t *)vctx;
+    ora_fexchem(ctx->rk, y, f);
+}
+
+static void stage_solve(void *vctx, const double *rhs, double *x)
+{
+    stage_ctx_t *ctx = (stage_ctx_t *)vctx;
+    if (ctx->D->has_sparse) {
+        double tmp[NS];
+        memcpy(tmp, rhs, sizeof tmp);
+        g_sp_solve(ctx->D->sp_handle, tmp, x);
+    } else {
+        ora_lu_solve(ctx->lu, rhs, x);
+    }
+}
+
+/* Linear test problem y' = A y (n <= 47, dense A row-major) integrated over
+ * nsteps fixed steps of size h with the same stage code; A is factored as
+ * 1/(gamma h) I - A by dense elimination.  err_out receives the error norm of
+ * the last step with atol = rtol = 1 (may be NULL). */
+typedef struct { int n; const double *A; double lu[NS * NS]; int piv[NS]; } lin_ctx_t;
+
+static void lin_fun(void *vctx, const double *y, double *f)
+{
+    lin_ctx_t *c = (lin_ctx_t *)vctx;
+    for (int i = 0; i < c->n; i++) {
+        double s = 0.0;
+        for (int j = 0; j < c->n; j++) s += c->A[i * c->n + j] * y[j];
+        f[i] = s;
+    }
+}
+
+static void lin_solve(void *vctx, const double *rhs, double *x)
+{
+    lin_ctx_t *c = (lin_ctx_t *)vctx;
+    int n = c->n;
+    double b[NS];
+    for (int i = 0; i < n; i++) b[i] = rhs[c->piv[i]];
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < i; k++) b[i] -= c->lu[i * n + k] * b[k];
+    for (int i = n - 1; i >= 0; i--) {
+        for (int k = i + 1; k < n; k++) b[i] -= c->lu[i * n + k] * b[k];
+        b[i] /= c->lu[i * n + i];
+    }
+    for (int i = 0; i < n; i++) x[i] = b[i];
+}
+
+int ora_rodas3_linear(int n, const double *A, const double *y0, double h, int nsteps,
+                      double *y_out, double *err_out)
+{
+    if (n < 1 || n > NS) return -1;
+    lin_ctx_t c;
+    c.n = n;
+    c.A = A;
+    /* dense LU with partial pivoting of 1/(gamma h) I - A */
+    for (int i = 0; i < n; i++) {
+        c.piv[i] = i;
+        for (int j = 0; j < n; j++) c.lu[i * n + j] = (i == j ? 1.0 / (0.5 * h) : 0.0) - A[i * n + j];
+    }
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        for (int i = k + 1; i < n; i++)
+            if (fabs(c.lu[i * n + k]) > fabs(c.lu[p * n + k])) p = i;
+        if (c.lu[p * n + k] == 0.0) return 1;
+        if (p != k) {
+            for (int j = 0; j < n; j++) {
+                double t = c.lu[k * n + j]; c.lu[k * n + j] = c.lu[p * n + j]; c.lu[p * n + j] = t;
+            }
+            int t = c.piv[k]; c.piv[k] = c.piv[p]; c.piv[p] = t;
+        }
+        for (int i = k + 1; i < n; i++) {
+            c.lu[i * n + k] /= c.lu[k * n + k];
+            for (int j = k + 1; j < n; j++) c.lu[i * n + j] -= c.lu[i * n + k] * c.lu[k * n + j];
+        }
+    }
+    double y[NS], f0[NS], k1[NS], k2[NS], k3[NS], k4[NS], yn[NS], ones[NS], err = 0.0;
+    for (int i = 0; i < n; i++) { y[i] = y0[i]; ones[i] = 1.0; }
+    for (int s = 0; s < nsteps; s++) {
+        lin_fun(&c, y, f0);
+        rodas3_stages(n, &c, lin_fun, lin_solve, y, f0, h, k1, k2, k3, k4, yn);
+        rodas3_solution(n, y, k1, k3, k4, ones, ones, yn, &err);
+        for (int i = 0; i < n; i++) y[i] = yn[i];
+    }
+    for (int i = 0; i < n; i++) y_out[i] = y[i];
+    if (err_out) *err_out = err;
+    return 0;
+}
+
 /* RODAS3 on one block, chem_spack_rodas3_dyndt.f90:413-705 */
 static int integrate_block(drv_t *D, blk_t *B, int b)
 {
     const brams_chem_args *a = D->a;
-    const double c83 = 8.0 / 3.0;
     const float tn = a->dtlt * (float)a->n_dyn_chem;     /* REAL product */
     double dt = a->last_accepted_dt[b];
     const double dt_min = fmax(1.0, 1.0e-2 * (double)tn);
@@ -507,13 +670,11 @@ static int integrate_block(drv_t *D, blk_t *B, int b)
                     g_sp_load_factor(D->sp_handle, nz, D->sp_elem, vals);
                 }
             }
-            const double dti = 1.0 / dt;
             for (int c = 0; c < B->n; c++) {
                 double mat[NS * NS];
                 for (int i = 0; i < NS; i++)
                     for (int j = 0; j < NS; j++)
                         mat[i * NS + j] = (i == j) ? igdt - B->jac[c][i * NS + j] : -B->jac[c][i * NS + j];
-                double rhs[NS];
                 if (D->has_sparse) {
                     double vals[NS * NS];
                     for (int z = 0; z < D->sp_nnz; z++)
@@ -523,55 +684,22 @@ static int integrate_block(drv_t *D, blk_t *B, int b)
                     ora_lu_load(&B->lu, mat, D->mask);
                     if (ora_lu_factor(&B->lu) != 0) failed = 1;
                 }
-#define SOLVE(in, out) do { if (D->has_sparse) { memcpy(rhs, in, sizeof rhs); g_sp_solve(D->sp_handle, rhs, out); } \
-                            else ora_lu_solve(&B->lu, in, out); } while (0)
-                double *y0 = B->y[c], *k1 = B->k1[c], *k2 = B->k2[c], *k3 = B->k3[c], *k4 = B->k4[c];
-                double *yn = B->ynew[c], bb[NS];
-                /* stage 1, rodas3:523-527 */
-                SOLVE(B->f0[c], k1);
-                /* stage 2, rodas3:534-538 */
-                for (int s = 0; s < NS; s++) bb[s] = (4.0 * dti) * k1[s] + B->f0[c][s];
-                SOLVE(bb, k2);
-                /* stage 3 with clipping, rodas3:542-566 */
-                for (int s = 0; s < NS; s++) {
-                    yn[s] = y0[s] + 2.0 * k1[s];
-                    if (yn[s] < 0.0) { yn[s] = 0.0; k1[s] = 0.5 * (yn[s] - y0[s]); }
-                }
-                ora_fexchem(B->rk[c], yn, B->ftmp);
-                for (int s = 0; s < NS; s++) bb[s] = B->ftmp[s] + dti * (k1[s] - k2[s]);
-                SOLVE(bb, k3);
-                /* stage 4 with clipping, rodas3:573-599 */
-                for (int s = 0; s < NS; s++) {
-                    yn[s] = y0[s] + 2.0 * k1[s] + k3[s];
-                    if (yn[s] < 0.0) { yn[s] = 0.0; k3[s] = (yn[s] - y0[s]) - 2.0 * k1[s]; }
-                }
-                ora_fexchem(B->rk[c], yn, B->ftmp);
-                for (int s = 0; s < NS; s++) bb[s] = B->ftmp[s] + dti * (k1[s] - k2[s] - c83 * k3[s]);
-                SOLVE(bb, k4);
-#undef SOLVE
+                stage_ctx_t ctx = {D, &B->lu, B->rk[c]};
+                rodas3_stages(NS, &ctx, stage_fun, stage_solve, B->y[c], B->f0[c], dt,
+                              B->k1[c], B->k2[c], B->k3[c], B->k4[c], B->ynew[c]);
             }
             /* solution and error, rodas3:609-648 */
             double max_err = 0.0;
             int all_accepted = 1;
             for (int c = 0; c < B->n; c++) {
-                double e = 0.0;
-                for (int s = 0; s < NS; s++) {
-                    double v = B->y[c][s] + 2.0 * B->k1[c][s] + B->k3[c][s] + B->k4[c][s];
-                    v = fmax(v, 0.0);
-                    B->ynew[c][s] = v;
-                    double tol = a->atol[s] + a->rtol[s] * fmax(fabs(B->y[c][s]), fabs(v));
-                    double q = B->k4[c][s] / tol;
-                    e = e + q * q;
-                }
-                B->err[c] = fmax(1.0e-15, sqrt(e / NS));
-                if (!(B->err[c] == B->err[c]) || isinf(B->err[c])) failed = 1;
+                double e = rodas3_solution(NS, B->y[c], B->k1[c], B->k3[c], B->k4[c], a->atol, a->rtol,
+                                           B->ynew[c], &B->err[c]);
+                if (!isfinite(e)) failed = 1;
                 if (B->err[c] - 1.0e-8 > 1.0) all_accepted = 0;
                 if (c == 0 || B->err[c] > max_err) max_err = B->err[c];
             }
             /* controller, rodas3:650-700 */
-            double fac = fmin(6.0, fmax(0.2, 0.9 / pow(max_err, 1.0 / 3.0)));
-            double dt_new = dt * fac;
-            dt_new = fmax(dt_min, fmin(dt_max, dt_new));
+            double dt_new = rodas3_dt_new(dt, max_err, dt_min, dt_max);
             if (failed) { time_c = time_f; break; }
             if (!all_accepted && dt_new > dt_min) {
                 dt = dt_new;

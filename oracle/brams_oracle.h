@@ -48,7 +48,15 @@ int ora_alloc_index(int block_size, int m1, int ia, int iz, int ja, int jz,
 int ora_rodas3_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
                      unsigned char *mask_out);
 
-/* Optional per-block trace of block b: attempts, sub-steps. */
+/* The RODAS3 stages, solution and error norm of the step applied to the
+ * linear problem y' = A y (dense n x n, n <= 47) with nsteps fixed steps h;
+ * err_out = error norm of the last step with atol = rtol = 1. */
+int ora_rodas3_linear(int n, const double *A, const double *y0, double h, int nsteps,
+                      double *y_out, double *err_out);
+/* The step-size controller: dt * min(6, max(0.2, 0.9 err^-1/3)) in [dt_min, dt_max]. */
+double ora_dt_new(double dt, double max_err, double dt_min, double dt_max);
+
+/* Per-block counts of the last step: attempts, accepted sub-steps. */
 int ora_block_counts(int64_t *attempts, int64_t *substeps, int max_nob);
 
 /* Route the LU through an external Sparse1.3 build (the reference's, via

@@ -48,7 +48,9 @@ class ChemArgs(C.Structure):
 class ChemStats(C.Structure):
     _fields_ = [("cells", C.c_int64), ("blocks", C.c_int64), ("substeps", C.c_int64),
                 ("attempts", C.c_int64), ("failed_blocks", C.c_int64),
-                ("mask_nnz", C.c_int32), ("kernel_ms", C.c_float)]
+                ("mask_nnz", C.c_int32), ("kernel_ms", C.c_float),
+                ("cell_substeps", C.c_int64), ("cell_attempts", C.c_int64),
+                ("block1_ms", C.c_float), ("main_ms", C.c_float), ("main_cells", C.c_int64)]
 
 
 _oracle = None
@@ -197,3 +199,95 @@ def oracle_lu_order():
     perm = np.zeros(NS, np.int32); nnz = np.zeros(1, np.int32)
     oracle().ora_lu_order(ptr(perm, _ip), ptr(nnz, _ip))
     return perm, int(nnz[0])
+
+
+# ------------------------------------------------------------- whole step
+class HostStep:
+    """Owns the numpy arrays behind one brams_chem_args and keeps them alive."""
+
+    def __init__(self, wl, atol, rtol, split_method=0, n_dyn_chem=1, sc_t=None, sc_t_dyn=None):
+        from brams_amd.mechanism import load_mechanism
+        mech = load_mechanism()
+        g = wl["grid"]
+        self.wl = wl
+        self.sc_p = [np.array(a, copy=True, order="F") for a in wl["sc_p"]]
+        self.last_dt = np.array(wl["last_accepted_dt"], dtype=np.float64, copy=True)
+        self.weight = np.array([s.weight for s in mech.species], dtype=np.float32)
+        self.tidx = np.array([i + 1 for i, s in enumerate(mech.species) if s.transported], np.int32)
+        self.nidx = np.array([i + 1 for i, s in enumerate(mech.species) if not s.transported], np.int32)
+        self.block_end = np.ascontiguousarray(wl["block_end"], dtype=np.int32)
+        self.atol = np.ascontiguousarray(atol, dtype=np.float64)
+        self.rtol = np.ascontiguousarray(rtol, dtype=np.float64)
+        self.sc_t = None if sc_t is None else [np.array(a, copy=True, order="F") for a in sc_t]
+        self.sc_t_dyn = None if sc_t_dyn is None else [np.array(a, copy=True, order="F") for a in sc_t_dyn]
+        a = ChemArgs()
+        a.m1, a.m2, a.m3 = g.m1, g.m2, g.m3
+        a.ia, a.iz, a.ja, a.jz = g.ia, g.iz, g.ja, g.jz
+        a.dtlt = g.dtlt
+        a.n_dyn_chem = n_dyn_chem
+        a.split_method = split_method
+        a.photoj_method = 0
+        a.cp, a.cpor, a.p00 = 1004.0, 1004.0 / 287.0, 1.0e5
+        a.pp, a.pi0 = ptr(wl["pp"], _fp), ptr(wl["pi0"], _fp)
+        a.theta, a.rv = ptr(wl["theta"], _fp), ptr(wl["rv"], _fp)
+        a.jphoto = ptr(wl["jphoto"])
+        a.nspecies, a.nr, a.nr_photo = NS, NR, NP
+        a.weight = ptr(self.weight, _fp)
+        a.n_transported, a.n_not_transported = len(self.tidx), len(self.nidx)
+        a.transp_chem_index = ptr(self.tidx, _ip)
+        a.no_transp_chem_index = ptr(self.nidx, _ip)
+        self._scp_tab = (_fp * NS)(*[ptr(x, _fp) for x in self.sc_p])
+        a.sc_p = C.cast(self._scp_tab, C.POINTER(_fp))
+        if self.sc_t is not None:
+            self._sct_tab = (_fp * NS)(*[ptr(x, _fp) for x in self.sc_t])
+            a.sc_t = C.cast(self._sct_tab, C.POINTER(_fp))
+        if self.sc_t_dyn is not None:
+            self._sctd_tab = (_fp * NS)(*[ptr(x, _fp) for x in self.sc_t_dyn])
+            a.sc_t_dyn = C.cast(self._sctd_tab, C.POINTER(_fp))
+        a.nob = len(self.block_end)
+        a.maxblock_size = int(self.block_end.max())
+        a.block_end = ptr(self.block_end, _ip)
+        a.last_accepted_dt = ptr(self.last_dt)
+        a.atol, a.rtol = ptr(self.atol), ptr(self.rtol)
+        self.args = a
+
+
+def run_oracle(wl, atol, rtol, nthreads=0, split_method=0, n_dyn_chem=1, sc_t=None, sc_t_dyn=None,
+               reference_sparse=False):
+    """Run the oracle step on a workload; returns (HostStep, stats, mask, rc)."""
+    hs = HostStep(wl, atol, rtol, split_method, n_dyn_chem, sc_t, sc_t_dyn)
+    st = ChemStats()
+    mask = np.zeros(NS * NS, np.uint8)
+    lib = oracle()
+    if reference_sparse:
+        use_reference_sparse(True)
+    try:
+        rc = lib.ora_rodas3_dyndt(C.byref(hs.args), nthreads, C.byref(st), ptr(mask, C.POINTER(C.c_uint8)))
+    finally:
+        if reference_sparse:
+            use_reference_sparse(False)
+    return hs, st, mask.reshape(NS, NS), rc
+
+
+def block_counts(nob):
+    a = np.zeros(nob, np.int64); s = np.zeros(nob, np.int64)
+    oracle().ora_block_counts(a.ctypes.data_as(C.POINTER(C.c_int64)),
+                              s.ctypes.data_as(C.POINTER(C.c_int64)), nob)
+    return a, s
+
+
+def rodas3_linear(A, y0, h, nsteps):
+    lib = oracle()
+    lib.ora_rodas3_linear.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp]
+    A = np.ascontiguousarray(A, np.float64); y0 = np.ascontiguousarray(y0, np.float64)
+    out = np.zeros(len(y0)); err = np.zeros(1)
+    rc = lib.ora_rodas3_linear(len(y0), ptr(A), ptr(y0), h, nsteps, ptr(out), ptr(err))
+    assert rc == 0
+    return out, float(err[0])
+
+
+def dt_new(dt, err, dt_min, dt_max):
+    lib = oracle()
+    lib.ora_dt_new.argtypes = [C.c_double] * 4
+    lib.ora_dt_new.restype = C.c_double
+    return lib.ora_dt_new(dt, err, dt_min, dt_max)
