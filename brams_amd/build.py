@@ -34,7 +34,7 @@ def gen():
 
 
 def build_hip():
-    srcs = [os.path.join(CSRC, "brams_chem.hip")]
+    srcs = [os.path.join(CSRC, "brams_chem.hip"), os.path.join(CSRC, "surrogate.hip")]
     tmp = LIB + ".tmp"
     run([HIPCC] + HIP_FLAGS + ["-I", CSRC, "-o", tmp] + srcs)
     os.replace(tmp, LIB)

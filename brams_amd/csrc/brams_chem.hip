@@ -197,6 +197,9 @@ int check_common(int m1, int m2, int m3, int ia, int iz, int ja, int jz, int ndy
 
 extern "C" {
 
+// shared by the surrogate translation unit: record an error, return code
+int brams_chem_set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
 int brams_chem_init(int device) {
   if (g.init && g.device == device) return 0;
   int n = 0;

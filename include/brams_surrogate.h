@@ -1,0 +1,66 @@
+/*
+ * brams_surrogate.h -- learned surrogate of the CCATT chemistry step.
+ *
+ * No counterpart in the reference (which only has the RODAS3 solver,
+ * src/brams/ccatt/chem_spack_rodas3_dyndt.f90).  A small MLP maps one cell's
+ * state (log10 concentrations of the 47 species, T, ln p, log10 rv, the 17
+ * photolysis rates, log10 dt) to the log10 increments of the 47 species over
+ * one chemistry step.  Weights and activations are BF16 on the matrix cores,
+ * accumulation and master weights FP32.  Cells whose prediction fails the
+ * checks (non-finite or negative state, nitrogen/carbon budget, ensemble
+ * spread) are integrated by the FP64 RODAS3 kernels instead.
+ *
+ * All array arguments are device pointers; calls are synchronous.
+ * Status: 0 ok, <0 error (text in brams_chem_last_error()).
+ */
+#ifndef BRAMS_SURROGATE_H
+#define BRAMS_SURROGATE_H
+
+#include <stdint.h>
+#include "brams_chem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BRAMS_SUR_NIN 68       /* features per cell */
+#define BRAMS_SUR_NIN_PAD 128  /* padded feature row (bf16) */
+#define BRAMS_SUR_NOUT 47      /* log10 increments */
+#define BRAMS_SUR_NOUT_PAD 64  /* padded output row (f32) */
+
+typedef struct brams_mlp brams_mlp;
+
+/* One network: nlayers hidden layers of width nhidden (multiple of 64) with
+ * GELU, input BRAMS_SUR_NIN_PAD, output BRAMS_SUR_NOUT_PAD; training batches
+ * up to max_batch rows (multiple of 64).  Weights are initialised
+ * U(-1/sqrt(fan_in), 1/sqrt(fan_in)) from seed. */
+int brams_mlp_create(int nhidden, int nlayers, int max_batch, uint64_t seed, brams_mlp **out);
+void brams_mlp_destroy(brams_mlp *m);
+/* number of FP32 master parameters (layer by layer: W[out][in] then b[out]) */
+int64_t brams_mlp_num_params(const brams_mlp *m);
+/* copy master parameters device->host / host->device (n floats) */
+int brams_mlp_get_params(const brams_mlp *m, float *host, int64_t n);
+int brams_mlp_set_params(brams_mlp *m, const float *host, int64_t n);
+
+/* Forward pass: X [M][BRAMS_SUR_NIN_PAD] bf16 -> out [M][BRAMS_SUR_NOUT_PAD]
+ * f32, M a multiple of 64 (any size; inference buffers are allocated here). */
+int brams_mlp_forward(brams_mlp *m, const void *X, int64_t M, float *out, void *stream);
+
+/* Gradient of the mean squared error over the first BRAMS_SUR_NOUT outputs
+ * of one batch (B <= max_batch): grads (host, num_params floats) receives
+ * dL/dparams, loss (host) the loss.  Parameters are not changed. */
+int brams_mlp_gradients(brams_mlp *m, const void *X, const float *Y, int B, float *grads,
+                        float *loss);
+
+/* Train on a resident dataset X [N][NIN_PAD] bf16, Y [N][NOUT_PAD] f32 for
+ * `epochs` passes in shuffled batches of `batch` rows (the last partial batch
+ * of an epoch is dropped), Adam (beta 0.9/0.999, eps 1e-8) with a cosine
+ * learning-rate decay from lr0 to 0.  epoch_loss (host, epochs floats, may be
+ * NULL) receives the mean training loss of each epoch. */
+int brams_mlp_train(brams_mlp *m, const void *X, const float *Y, int64_t N, int batch, int epochs,
+                    float lr0, uint64_t seed, float *epoch_loss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

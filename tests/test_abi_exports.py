@@ -1,0 +1,54 @@
+"""The C-ABI library loads without a GPU and exports every function that
+include/*.h declares; host-only entry points behave like the reference."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from brams_amd import abi
+from brams_amd.workload import Grid, alloc_index
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(brams_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_headers_declare_the_expected_entry_points():
+    names = _declared()
+    for n in abi.EXPORTS + ["brams_mlp_create", "brams_mlp_train", "brams_mlp_forward",
+                            "brams_mlp_gradients"]:
+        assert n in names, n
+
+
+@pytest.mark.parametrize("name", _declared())
+def test_library_exports(name):
+    lib = C.CDLL(abi.LIB_PATH)
+    assert hasattr(lib, name), name
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(16, 16, 10), (98, 98, 40), (50, 50, 40), (3, 3, 12), (1, 1, 5)])
+def test_alloc_index_through_c_abi(nx, ny, nz):
+    from brams_amd.chem import alloc_index as c_alloc
+    g = Grid(nx, ny, nz)
+    assert np.array_equal(c_alloc(g.m1, g.ia, g.iz, g.ja, g.jz), alloc_index(g))
+
+
+def test_invalid_arguments_are_reported_not_aborted():
+    L = abi.lib()
+    # a host step with inconsistent block partition must return an error code
+    a = abi.ChemArgs()
+    a.m1, a.m2, a.m3, a.ia, a.iz, a.ja, a.jz = 12, 18, 18, 2, 17, 2, 17
+    rc = L.brams_chem_alloc_index(64, 2, 2, 1, 2, 17, None, 0)
+    assert rc < 0                      # m1=2 -> no interior levels
+    assert b"empty" in L.brams_chem_last_error()
