@@ -17,8 +17,12 @@
 #include <string>
 #include <vector>
 
+#include <algorithm>
+
 #include "../../include/brams_chem.h"
+#include "../../include/brams_surrogate.h"
 #include "rodas3_kernel.h"
+#include "sur_step_kernels.h"
 
 namespace {
 
@@ -69,6 +73,8 @@ struct State {
   DevBuf block_start, mask, attempts, substeps, failed;
   // host-entry staging
   DevBuf sc_p, sc_t, sc_t_dyn, met, jphoto, last_dt, consts;
+  // surrogate step
+  DevBuf sur_X, sur_out, sur_flag, sur_list, sur_chunk, sur_reason, fb_dt, fb_att, mask_all;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evb = nullptr, evm = nullptr;
   long long main_cells = 0;
   brams_chem_stats stats{};
@@ -233,7 +239,12 @@ int brams_chem_alloc_index(int32_t block_size, int32_t m1, int32_t ia, int32_t i
   return nob;
 }
 
-int brams_chem_rodas3_dyndt_device(const brams_chem_dev_args* a) {
+}  // extern "C"
+
+namespace {
+
+// validate device arguments and fill the kernel parameters (no launch)
+int prepare_device(const brams_chem_dev_args* a, brams::StepParams& p, long long& ncell) {
   if (!a) return fail(-2, "null args");
   if (!g.init) {
     int rc = brams_chem_init(0);
@@ -249,11 +260,11 @@ int brams_chem_rodas3_dyndt_device(const brams_chem_dev_args* a) {
     return fail(-2, "PARALLEL with N_DYN_CHEM=1 needs sc_t");
   if (a->split_method == BRAMS_SPLIT_PARALLEL && a->n_dyn_chem > 1 && !a->sc_t_dyn)
     return fail(-2, "PARALLEL with N_DYN_CHEM>1 needs sc_t_dyn");
-  const long long ncell = (long long)(a->m1 - 2) * (a->iz - a->ia + 1) * (a->jz - a->ja + 1);
+  ncell = (long long)(a->m1 - 2) * (a->iz - a->ia + 1) * (a->jz - a->ja + 1);
   if ((rc = ensure_partition(ncell))) return rc;
   const int nob = (int)g.part_sizes.size();
   if (a->nob != nob) return fail(-2, "nob %d does not match the AllocIndex partition (%d)", a->nob, nob);
-  brams::StepParams p{};
+  p = brams::StepParams{};
   p.m1 = a->m1; p.m2 = a->m2; p.m3 = a->m3;
   p.ia = a->ia; p.iz = a->iz; p.ja = a->ja; p.jz = a->jz;
   p.nk = a->m1 - 2; p.ni = a->iz - a->ia + 1;
@@ -266,9 +277,185 @@ int brams_chem_rodas3_dyndt_device(const brams_chem_dev_args* a) {
   p.n3d = (long long)a->m1 * a->m2 * a->m3;
   p.last_dt = a->last_accepted_dt;
   p.atol = a->atol; p.rtol = a->rtol;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int brams_chem_rodas3_dyndt_device(const brams_chem_dev_args* a) {
+  brams::StepParams p;
+  long long ncell = 0;
+  int rc = prepare_device(a, p, ncell);
+  if (rc) return rc;
+  const int nob = (int)g.part_sizes.size();
   hipStream_t st = (hipStream_t)a->stream;
   if ((rc = launch_step(p, st, nob))) return rc;
   return collect_stats(st, nob, ncell);
+}
+
+// ------------------------------------------------------------------ surrogate
+int brams_sur_features_raw(const brams_chem_dev_args* a, float* feat, double floor_c) {
+  brams::StepParams p;
+  long long ncell = 0;
+  int rc = prepare_device(a, p, ncell);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)a->stream;
+  hipLaunchKernelGGL(brams::sur_raw_features, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st,
+                     p, ncell, floor_c, feat);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  return 0;
+}
+
+int brams_sur_targets_raw(const brams_chem_dev_args* a, const float* sc_p_before, float* targ,
+                          double floor_c) {
+  brams::StepParams p;
+  long long ncell = 0;
+  int rc = prepare_device(a, p, ncell);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)a->stream;
+  hipLaunchKernelGGL(brams::sur_raw_targets, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st,
+                     p, ncell, floor_c, sc_p_before, targ);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  return 0;
+}
+
+int brams_sur_moments(const float* x, int64_t n, int ncol, double* sum, double* sumsq) {
+  if (!x || n <= 0 || ncol <= 0) return fail(-2, "bad moments arguments");
+  HIP_TRY(hipMemset(sum, 0, sizeof(double) * ncol));
+  HIP_TRY(hipMemset(sumsq, 0, sizeof(double) * ncol));
+  const int blocks = (int)std::min<int64_t>(2048, (n + 3) / 4);
+  hipLaunchKernelGGL(brams::column_moments, dim3(blocks), dim3(256), 0, nullptr, x, (long long)n, ncol,
+                     sum, sumsq);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  return 0;
+}
+
+int brams_sur_normalize(const float* raw, int64_t n, int ncol, const float* mu, const float* sd,
+                        void* out, int ldo, int out_bf16) {
+  if (!raw || !out || n <= 0 || ncol > ldo) return fail(-2, "bad normalize arguments");
+  const long long tot = (long long)n * ldo;
+  const unsigned blocks = (unsigned)((tot + 255) / 256);
+  if (out_bf16)
+    hipLaunchKernelGGL(brams::normalize_rows<true>, dim3(blocks), dim3(256), 0, nullptr, raw,
+                       (long long)n, ncol, mu, sd, out, ldo);
+  else
+    hipLaunchKernelGGL(brams::normalize_rows<false>, dim3(blocks), dim3(256), 0, nullptr, raw,
+                       (long long)n, ncol, mu, sd, out, ldo);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  return 0;
+}
+
+int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams_sur_stats* out) {
+  brams::StepParams p;
+  long long ncell = 0;
+  int rc = prepare_device(a, p, ncell);
+  if (rc) return rc;
+  if (!m || m->nmembers < 1 || m->nmembers > 4) return fail(-2, "surrogate needs 1..4 members");
+  hipStream_t st = (hipStream_t)a->stream;
+  const long long nrows = (ncell + 63) / 64 * 64;
+  const long long nchunk = (ncell + 1023) / 1024;
+  const long long maxfb = (ncell + 63) / 64;
+  HIP_TRY(g.sur_X.ensure(sizeof(__bf16) * nrows * brams::kSurNinPad));
+  HIP_TRY(g.sur_out.ensure(sizeof(float) * nrows * brams::kSurNoutPad * m->nmembers));
+  HIP_TRY(g.sur_flag.ensure(sizeof(int) * ncell));
+  HIP_TRY(g.sur_list.ensure(sizeof(int) * ncell));
+  HIP_TRY(g.sur_chunk.ensure(sizeof(int) * (2 * nchunk + 2)));
+  HIP_TRY(g.sur_reason.ensure(sizeof(int) * 4));
+  HIP_TRY(g.fb_dt.ensure(sizeof(double) * maxfb));
+  HIP_TRY(g.fb_att.ensure(sizeof(int) * maxfb * 3));
+  HIP_TRY(g.mask_all.ensure(sizeof(unsigned) * MECH_MASK_WORDS));
+  hipEvent_t ev[5];
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipEventRecord(ev[0], st));
+  // 1. features
+  hipLaunchKernelGGL(brams::sur_features, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, st, p,
+                     ncell, nrows, m->floor_c, m->feat_mu, m->feat_sd, g.sur_X.as<__bf16>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ev[1], st));
+  // 2. ensemble inference on the matrix cores
+  brams::SurApplyParams ap{};
+  for (int k = 0; k < m->nmembers; ++k) {
+    float* o = g.sur_out.as<float>() + (long long)k * nrows * brams::kSurNoutPad;
+    rc = brams_mlp_forward(m->members[k], g.sur_X.p, nrows, o, st);
+    if (rc) return rc;
+    ap.out[k] = o;
+  }
+  HIP_TRY(hipEventRecord(ev[2], st));
+  // 3. checks, write-back of accepted cells, ordered list of rejected ones
+  ap.nmembers = m->nmembers;
+  ap.tmu = m->targ_mu; ap.tsd = m->targ_sd;
+  ap.natoms = m->natoms; ap.catoms = m->catoms;
+  ap.floor_c = m->floor_c; ap.sig_c = m->sig_c;
+  ap.spread_tol = m->spread_tol; ap.n_tol = m->n_tol; ap.c_tol = m->c_tol;
+  ap.flag = g.sur_flag.as<int>();
+  ap.reason = g.sur_reason.as<int>();
+  HIP_TRY(hipMemsetAsync(ap.reason, 0, sizeof(int) * 4, st));
+  hipLaunchKernelGGL(brams::sur_apply, dim3((unsigned)((ncell + 127) / 128)), dim3(128), 0, st, p, ap,
+                     ncell);
+  HIP_TRY(hipGetLastError());
+  int* cnt = g.sur_chunk.as<int>();
+  int* off = cnt + nchunk;
+  hipLaunchKernelGGL(brams::compact_count, dim3((unsigned)nchunk), dim3(256), 0, st, ap.flag, ncell, cnt);
+  hipLaunchKernelGGL(brams::compact_scan, dim3(1), dim3(256), 0, st, cnt, (int)nchunk, off);
+  hipLaunchKernelGGL(brams::compact_write, dim3((unsigned)nchunk), dim3(256), 0, st, ap.flag, ncell, off,
+                     g.sur_list.as<int>());
+  HIP_TRY(hipGetLastError());
+  int nfb = 0;
+  int reasons[4];
+  HIP_TRY(hipMemcpyAsync(&nfb, off + nchunk, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(reasons, ap.reason, sizeof reasons, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipEventRecord(ev[3], st));
+  HIP_TRY(hipStreamSynchronize(st));
+  // 4. RODAS3 on the rejected cells, in blocks of 64 consecutive list entries
+  //    sharing one step (full structural Jacobian, fresh step dtlt*N)
+  long long fb_att = 0;
+  if (nfb > 0) {
+    const int nb = (nfb + 63) / 64;
+    std::vector<double> dt0(nb, (double)(a->dtlt * (float)a->n_dyn_chem));
+    HIP_TRY(hipMemcpyAsync(g.fb_dt.p, dt0.data(), sizeof(double) * nb, hipMemcpyHostToDevice, st));
+    unsigned ones[MECH_MASK_WORDS];
+    for (auto& w : ones) w = 0xffffffffu;
+    HIP_TRY(hipMemcpyAsync(g.mask_all.p, ones, sizeof ones, hipMemcpyHostToDevice, st));
+    brams::StepParams q = p;
+    q.cell_list = g.sur_list.as<int>();
+    q.list_n = nfb;
+    q.last_dt = g.fb_dt.as<double>();
+    q.mask = g.mask_all.as<unsigned>();
+    q.blk_attempts = g.fb_att.as<int>();
+    q.blk_substeps = q.blk_attempts + maxfb;
+    q.blk_failed = q.blk_attempts + 2 * maxfb;
+    q.block_start = nullptr;
+    hipLaunchKernelGGL((brams::rodas3_blocks<64, false>), dim3(nb), dim3(64), 0, st, q, 0);
+    HIP_TRY(hipGetLastError());
+    std::vector<int> att(nb);
+    HIP_TRY(hipMemcpyAsync(att.data(), q.blk_attempts, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int b = 0; b < nb; ++b) fb_att += att[b];
+  }
+  HIP_TRY(hipEventRecord(ev[4], st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (out) {
+    brams_sur_stats s{};
+    s.cells = ncell;
+    s.fallback_cells = nfb;
+    for (int k = 0; k < 4; ++k) s.reasons[k] = reasons[k];
+    s.fallback_block_attempts = fb_att;
+    float ms;
+    HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); s.features_ms = ms;
+    HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[2])); s.mlp_ms = ms;
+    HIP_TRY(hipEventElapsedTime(&ms, ev[2], ev[3])); s.apply_ms = ms;
+    HIP_TRY(hipEventElapsedTime(&ms, ev[3], ev[4])); s.fallback_ms = ms;
+    HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[4])); s.total_ms = ms;
+    *out = s;
+  }
+  for (auto& e : ev) HIP_TRY(hipEventDestroy(e));
+  return 0;
 }
 
 int brams_chem_rodas3_dyndt(const brams_chem_args* a) {
@@ -393,7 +580,9 @@ const char* brams_chem_last_error(void) { return g_err.c_str(); }
 
 void brams_chem_finalize(void) {
   DevBuf* bufs[] = {&g.block_start, &g.mask, &g.attempts, &g.substeps, &g.failed, &g.sc_p,
-                    &g.sc_t, &g.sc_t_dyn, &g.met, &g.jphoto, &g.last_dt, &g.consts};
+                    &g.sc_t, &g.sc_t_dyn, &g.met, &g.jphoto, &g.last_dt, &g.consts,
+                    &g.sur_X, &g.sur_out, &g.sur_flag, &g.sur_list, &g.sur_chunk,
+                    &g.sur_reason, &g.fb_dt, &g.fb_att, &g.mask_all};
   for (DevBuf* b : bufs) b->release();
   hipEvent_t* evs[] = {&g.ev0, &g.ev1, &g.evb, &g.evm};
   for (hipEvent_t* e : evs) {

@@ -41,6 +41,8 @@ struct StepParams {
   int* blk_attempts;             // [nob]
   int* blk_substeps;             // [nob]
   int* blk_failed;               // [nob]
+  const int* cell_list;          // optional: blocks are consecutive WG-sized runs of this list
+  int list_n;                    // length of cell_list
 };
 
 enum { kSplitSymmetric = 0, kSplitSequential = 1, kSplitParallel = 2 };
@@ -83,11 +85,18 @@ __device__ __forceinline__ void block_reduce(double& errmax, int& rej, int& fail
 template <int WG, bool DYN>
 __global__ __launch_bounds__(WG) void rodas3_blocks(StepParams p, int block0) {
   const int b = block0 + blockIdx.x;
-  const int start = p.block_start[b];
-  const int ncell = p.block_start[b + 1] - start;
+  int start, ncell;
+  if (p.cell_list) {                                // fallback blocks of listed cells
+    start = b * WG;
+    ncell = min(WG, p.list_n - start);
+  } else {
+    start = p.block_start[b];
+    ncell = p.block_start[b + 1] - start;
+  }
   const int c = threadIdx.x;
   const bool act = c < ncell;
-  const long long g = start + (act ? c : 0);        // idle lanes shadow cell 1
+  const long long g = p.cell_list ? (long long)p.cell_list[start + (act ? c : 0)]
+                                  : (long long)(start + (act ? c : 0));  // idle lanes shadow cell 1
 
   // cell position, j -> i -> k order (chem_spack_utils.f90:144-160)
   const long long kk = 2 + g % p.nk;

@@ -60,6 +60,46 @@ int brams_mlp_gradients(brams_mlp *m, const void *X, const float *Y, int B, floa
 int brams_mlp_train(brams_mlp *m, const void *X, const float *Y, int64_t N, int batch, int epochs,
                     float lr0, uint64_t seed, float *epoch_loss);
 
+/* ------------------------------------------------------------------ step
+ * The surrogate chemistry step on one grid, device-resident arrays as in
+ * brams_chem_rodas3_dyndt_device (SYMMETRIC/SEQUENTIAL write-back). */
+typedef struct brams_sur_model {
+    brams_mlp *members[4];           /* ensemble */
+    int32_t nmembers;
+    const float *feat_mu, *feat_sd;  /* device [68] feature normalisation */
+    const float *targ_mu, *targ_sd;  /* device [47]; targ_sd = 0 marks species the
+                                        mechanism never changes (kept as they are) */
+    const float *natoms, *catoms;    /* device [47] N and C atoms per molecule */
+    double floor_c;                  /* floor (molec/cm^3) of the log10 features */
+    double sig_c;                    /* species below this before and after skip the spread check */
+    float spread_tol;                /* max relative ensemble spread of any species */
+    float n_tol, c_tol;              /* max relative change of the N / C budgets */
+} brams_sur_model;
+
+typedef struct brams_sur_stats {
+    int64_t cells;
+    int64_t fallback_cells;          /* cells integrated by RODAS3 */
+    int32_t reasons[4];              /* non-finite, N budget, C budget, spread */
+    int64_t fallback_block_attempts; /* RODAS3 attempts summed over fallback blocks */
+    float features_ms, mlp_ms, apply_ms, fallback_ms, total_ms;
+} brams_sur_stats;
+
+/* Raw features [ncell][BRAMS_SUR_NIN] f32 of the current state (dataset). */
+int brams_sur_features_raw(const brams_chem_dev_args *a, float *feat, double floor_c);
+/* Raw targets [ncell][BRAMS_SUR_NOUT] f32: log10 increments from sc_p_before
+ * (same layout as a->sc_p) to the current state a->sc_p (dataset). */
+int brams_sur_targets_raw(const brams_chem_dev_args *a, const float *sc_p_before, float *targ,
+                          double floor_c);
+/* Column sums and sums of squares (FP64, device) of x [n][ncol] f32. */
+int brams_sur_moments(const float *x, int64_t n, int ncol, double *sum, double *sumsq);
+/* z-score columns of raw [n][ncol] into out [n][ldo] (bf16 if out_bf16, else f32),
+ * padding columns set to 0. */
+int brams_sur_normalize(const float *raw, int64_t n, int ncol, const float *mu, const float *sd,
+                        void *out, int ldo, int out_bf16);
+/* Surrogate + fallback step: features, ensemble inference, checks, write-back
+ * of accepted cells, FP64 RODAS3 on the rejected ones. */
+int brams_sur_step(const brams_chem_dev_args *a, const brams_sur_model *m, brams_sur_stats *st);
+
 #ifdef __cplusplus
 }
 #endif
