@@ -20,7 +20,7 @@ _ip = C.POINTER(C.c_int32)
 # symbols include/brams_chem.h declares
 EXPORTS = ["brams_chem_init", "brams_chem_rodas3_dyndt", "brams_chem_rodas3_dyndt_device",
            "brams_chem_alloc_index", "brams_chem_get_stats", "brams_chem_last_error",
-           "brams_chem_finalize"]
+           "brams_chem_finalize", "brams_chem_block_status"]
 
 
 class ChemArgs(C.Structure):
@@ -69,7 +69,8 @@ class ChemStats(C.Structure):
                 ("attempts", C.c_int64), ("failed_blocks", C.c_int64),
                 ("mask_nnz", C.c_int32), ("kernel_ms", C.c_float),
                 ("cell_substeps", C.c_int64), ("cell_attempts", C.c_int64),
-                ("block1_ms", C.c_float), ("main_ms", C.c_float), ("main_cells", C.c_int64)]
+                ("block1_ms", C.c_float), ("main_ms", C.c_float), ("main_cells", C.c_int64),
+                ("forced_blocks", C.c_int64), ("max_forced_err", C.c_float)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -97,6 +98,7 @@ def lib():
         L.brams_chem_get_stats.argtypes = [C.POINTER(ChemStats)]
         L.brams_chem_last_error.restype = C.c_char_p
         L.brams_chem_finalize.argtypes = []
+        L.brams_chem_block_status.argtypes = [_ip, _ip, _ip, C.POINTER(C.c_float), C.c_int32]
         _lib = L
     return _lib
 
@@ -106,6 +108,17 @@ def check(rc, what="call"):
     if rc < 0:
         raise ChemError("%s failed (%d): %s" % (what, rc, lib().brams_chem_last_error().decode()))
     return rc
+
+
+def block_status(nob):
+    """Per-block (attempts, substeps, failed, forced_err) of the last step."""
+    import numpy as np
+    a = np.zeros(nob, np.int32); s = np.zeros(nob, np.int32); f = np.zeros(nob, np.int32)
+    e = np.zeros(nob, np.float32)
+    check(lib().brams_chem_block_status(a.ctypes.data_as(_ip), s.ctypes.data_as(_ip),
+                                        f.ctypes.data_as(_ip), e.ctypes.data_as(C.POINTER(C.c_float)),
+                                        nob), "brams_chem_block_status")
+    return a, s, f, e
 
 
 def stats():

@@ -70,7 +70,7 @@ struct State {
   bool init = false;
   int device = 0;
   // persistent per-call buffers
-  DevBuf block_start, mask, attempts, substeps, failed;
+  DevBuf block_start, mask, attempts, substeps, failed, forced;
   // host-entry staging
   DevBuf sc_p, sc_t, sc_t_dyn, met, jphoto, last_dt, consts;
   // surrogate step
@@ -81,6 +81,7 @@ struct State {
   std::vector<int> part_sizes;   // block sizes of the cached partition
   long long part_ncell = -1;
   std::vector<int> h_att, h_sub, h_fail;
+  std::vector<float> h_forced;
 };
 State g;
 
@@ -111,6 +112,7 @@ int ensure_partition(long long ncell) {
   HIP_TRY(g.attempts.ensure(sizeof(int) * nob));
   HIP_TRY(g.substeps.ensure(sizeof(int) * nob));
   HIP_TRY(g.failed.ensure(sizeof(int) * nob));
+  HIP_TRY(g.forced.ensure(sizeof(float) * nob));
   g.part_ncell = ncell;
   return 0;
 }
@@ -123,6 +125,7 @@ int launch_step(brams::StepParams p, hipStream_t st, int nob) {
   p.blk_attempts = g.attempts.as<int>();
   p.blk_substeps = g.substeps.as<int>();
   p.blk_failed = g.failed.as<int>();
+  p.blk_forced = g.forced.as<float>();
   const std::vector<int>& sz = g.part_sizes;
   int nbig = 0;
   while (nbig < nob && sz[nbig] > 64) ++nbig;
@@ -155,10 +158,12 @@ int collect_stats(hipStream_t st, int nob, long long ncell) {
   g.h_att.resize(nob);
   g.h_sub.resize(nob);
   g.h_fail.resize(nob);
+  g.h_forced.resize(nob);
   unsigned mw[MECH_MASK_WORDS];
   HIP_TRY(hipMemcpy(g.h_att.data(), g.attempts.p, sizeof(int) * nob, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(g.h_sub.data(), g.substeps.p, sizeof(int) * nob, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(g.h_fail.data(), g.failed.p, sizeof(int) * nob, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(g.h_forced.data(), g.forced.p, sizeof(float) * nob, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(mw, g.mask.p, sizeof mw, hipMemcpyDeviceToHost));
   brams_chem_stats s{};
   s.cells = ncell;
@@ -169,6 +174,8 @@ int collect_stats(hipStream_t st, int nob, long long ncell) {
     s.failed_blocks += g.h_fail[b];
     s.cell_attempts += (long long)g.h_att[b] * g.part_sizes[b];
     s.cell_substeps += (long long)g.h_sub[b] * g.part_sizes[b];
+    if (g.h_forced[b] > 1.0f) ++s.forced_blocks;
+    s.max_forced_err = std::max(s.max_forced_err, g.h_forced[b]);
   }
   int nz = 0;
   for (int w = 0; w < MECH_MASK_WORDS; ++w) nz += __builtin_popcount(mw[w]);
@@ -430,6 +437,7 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
     q.blk_attempts = g.fb_att.as<int>();
     q.blk_substeps = q.blk_attempts + maxfb;
     q.blk_failed = q.blk_attempts + 2 * maxfb;
+    q.blk_forced = nullptr;
     q.block_start = nullptr;
     hipLaunchKernelGGL((brams::rodas3_blocks<64, false>), dim3(nb), dim3(64), 0, st, q, 0);
     HIP_TRY(hipGetLastError());
@@ -570,6 +578,19 @@ int brams_chem_rodas3_dyndt(const brams_chem_args* a) {
   return collect_stats(st, nob, ncell);
 }
 
+int brams_chem_block_status(int32_t* attempts, int32_t* substeps, int32_t* failed,
+                            float* forced_err, int32_t max_nob) {
+  const int nob = (int)g.h_att.size();
+  if (nob > max_nob) return fail(-2, "max_nob %d < nob %d", max_nob, nob);
+  for (int b = 0; b < nob; ++b) {
+    if (attempts) attempts[b] = g.h_att[b];
+    if (substeps) substeps[b] = g.h_sub[b];
+    if (failed) failed[b] = g.h_fail[b];
+    if (forced_err) forced_err[b] = g.h_forced[b];
+  }
+  return nob;
+}
+
 int brams_chem_get_stats(brams_chem_stats* out) {
   if (!out) return fail(-2, "null stats");
   *out = g.stats;
@@ -579,7 +600,7 @@ int brams_chem_get_stats(brams_chem_stats* out) {
 const char* brams_chem_last_error(void) { return g_err.c_str(); }
 
 void brams_chem_finalize(void) {
-  DevBuf* bufs[] = {&g.block_start, &g.mask, &g.attempts, &g.substeps, &g.failed, &g.sc_p,
+  DevBuf* bufs[] = {&g.block_start, &g.mask, &g.attempts, &g.substeps, &g.failed, &g.forced, &g.sc_p,
                     &g.sc_t, &g.sc_t_dyn, &g.met, &g.jphoto, &g.last_dt, &g.consts,
                     &g.sur_X, &g.sur_out, &g.sur_flag, &g.sur_list, &g.sur_chunk,
                     &g.sur_reason, &g.fb_dt, &g.fb_att, &g.mask_all};

@@ -41,6 +41,7 @@ struct StepParams {
   int* blk_attempts;             // [nob]
   int* blk_substeps;             // [nob]
   int* blk_failed;               // [nob]
+  float* blk_forced;             // [nob] largest error of an acceptance forced at dt_min (0: none)
   const int* cell_list;          // optional: blocks are consecutive WG-sized runs of this list
   int list_n;                    // length of cell_list
 };
@@ -147,6 +148,7 @@ __global__ __launch_bounds__(WG) void rodas3_blocks(StepParams p, int block0) {
   double last_dt = dt;
   double time_c = 0.0;
   int natt = 0, nsub = 0, failed = 0;
+  float forced = 0.f;
 
   __shared__ unsigned s_mask[MECH_MASK_WORDS];
   __shared__ double s_err[WG / 64 + 1];
@@ -250,6 +252,8 @@ __global__ __launch_bounds__(WG) void rodas3_blocks(StepParams p, int block0) {
         dt = dt_new;
         continue;
       }
+      // an error above 1 accepted anyway: dt reached dt_min (rodas3:671-676)
+      if (rej) forced = fmaxf(forced, (float)errmax);
       time_c = time_c + dt;
       dt = fmin(dt_new, time_f - time_c);
       if (time_c < time_f) last_dt = dt_new;
@@ -289,6 +293,7 @@ __global__ __launch_bounds__(WG) void rodas3_blocks(StepParams p, int block0) {
     p.blk_attempts[b] = natt;
     p.blk_substeps[b] = nsub;
     p.blk_failed[b] = failed;
+    if (p.blk_forced) p.blk_forced[b] = forced;
     if constexpr (DYN) {
 #pragma unroll
       for (int w = 0; w < MECH_MASK_WORDS; ++w) p.mask[w] = s_mask[w];

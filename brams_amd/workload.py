@@ -94,6 +94,25 @@ def alloc_index(grid, block_size=64):
     return be
 
 
+def jphoto_from_sza(grid, sza_col):
+    """jphoto (17,m1,m2,m3) float64 for per-column solar zenith angles (deg):
+    CLEAR_SKY_J * cos(SZA), zero beyond 90 deg, with the RELACS relations for
+    slots 14 and 16; levels 1 and m1 copy their neighbours."""
+    m1, m2, m3 = grid.shape
+    nk, ni, nj = grid.nz, grid.nx, grid.ny
+    sza_col = np.asarray(sza_col, dtype=np.float64)
+    cosz = np.where(sza_col < 90.0, np.cos(np.deg2rad(sza_col)), 0.0)
+    J = CLEAR_SKY_J[None, None, None, :] * cosz[None, :, :, None]     # (1, ni, nj, 17)
+    J = np.broadcast_to(J, (nk, ni, nj, 17)).copy()
+    J[..., 13] = 0.962055 * J[..., 12] + 0.0106247 * J[..., 8]
+    J[..., 15] = 12.0 * J[..., 12] + 208.0 * J[..., 14]
+    jph = np.zeros((17, m1, m2, m3), dtype=np.float64, order="F")
+    jph[:, 1:m1 - 1, grid.ia - 1:grid.iz, grid.ja - 1:grid.jz] = np.moveaxis(J, 3, 0)
+    jph[:, 0] = jph[:, 1]
+    jph[:, m1 - 1] = jph[:, m1 - 2]
+    return jph
+
+
 def make_workload(grid, seed=0, sza="random", plume_sigma=None, plume_factor=50.0,
                   sza_offset=0.0):
     """Return a dict with the host arrays of one chemistry call on `grid`."""
@@ -118,7 +137,6 @@ def make_workload(grid, seed=0, sza="random", plume_sigma=None, plume_factor=50.
     else:
         raise ValueError(sza)
     sza_col = sza_col + sza_offset
-    cosz = np.where(sza_col < 90.0, np.cos(np.deg2rad(sza_col)), 0.0)
     # plume multiplier per column
     if plume_sigma:
         ii, jj = np.meshgrid(np.arange(ni), np.arange(nj), indexing="ij")
@@ -153,15 +171,7 @@ def make_workload(grid, seed=0, sza="random", plume_sigma=None, plume_factor=50.
             val = ppbv * 1.0e-9 * M
         sc_p.append(full(val, np.float32))
     wl["sc_p"] = sc_p
-    jmax = CLEAR_SKY_J
-    J = jmax[None, None, None, :] * cosz[None, :, :, None]            # (1, ni, nj, 17)
-    J = np.broadcast_to(J, (nk, ni, nj, 17)).copy()
-    J[..., 13] = 0.962055 * J[..., 12] + 0.0106247 * J[..., 8]
-    J[..., 15] = 12.0 * J[..., 12] + 208.0 * J[..., 14]
-    jph = np.zeros((17, m1, m2, m3), dtype=np.float64, order="F")
-    jph[:, 1:m1 - 1, grid.ia - 1:grid.iz, grid.ja - 1:grid.jz] = np.moveaxis(J, 3, 0)
-    jph[:, 0] = jph[:, 1]
-    jph[:, m1 - 1] = jph[:, m1 - 2]
+    jph = jphoto_from_sza(grid, sza_col)
     wl["jphoto"] = jph
     wl["block_end"] = alloc_index(grid)
     wl["last_accepted_dt"] = np.full(len(wl["block_end"]), float(np.float32(grid.dtlt)), dtype=np.float64)

@@ -109,6 +109,8 @@ typedef struct brams_chem_stats {
     float block1_ms;             /* device time of the block-1 kernel (mask, serial prefix) */
     float main_ms;               /* device time of the wave64 kernel over the 64-cell blocks */
     int64_t main_cells;          /* cells that kernel integrated */
+    int64_t forced_blocks;       /* blocks that accepted a step with error > 1 at dt_min */
+    float max_forced_err;        /* largest such error (rodas3:671-676 accepts it) */
 } brams_chem_stats;
 
 /* Select the HIP device and load the mechanism tables.  Idempotent. */
@@ -136,6 +138,12 @@ int brams_chem_alloc_index(int32_t block_size, int32_t m1, int32_t ia, int32_t i
 
 /* Counters of the last step. */
 int brams_chem_get_stats(brams_chem_stats *out);
+
+/* Per-block status of the last step (host arrays, any may be NULL): attempts,
+ * accepted sub-steps, failure flag, largest forced-acceptance error.  Returns
+ * nob. */
+int brams_chem_block_status(int32_t *attempts, int32_t *substeps, int32_t *failed,
+                            float *forced_err, int32_t max_nob);
 
 /* Text of the last error (thread-local, never NULL). */
 const char *brams_chem_last_error(void);

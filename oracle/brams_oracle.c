@@ -362,6 +362,7 @@ typedef struct {
     unsigned char mask[NS * NS]; /* value-based sparsity mask of block 1 */
     int64_t *attempts, *substeps;
     int *failed;
+    float *forced;
     atomic_int next;
     int has_sparse;              /* external Sparse1.3 in use */
     long sp_handle;
@@ -382,6 +383,7 @@ typedef struct {
 } blk_t;
 
 static int64_t *g_last_attempts, *g_last_substeps;
+static float *g_last_forced;
 static int g_last_nob;
 
 static void cell_offset(const brams_chem_args *a, long g, long *off)
@@ -639,6 +641,7 @@ static int integrate_block(drv_t *D, blk_t *B, int b)
     const double time_f = (double)tn;
     int failed = 0;
     int64_t natt = 0, nsub = 0;
+    float forced = 0.f;
 
     for (int c = 0; c < B->n; c++)
         ora_kinetic(B->temp[c], B->press[c], B->vapp[c], B->jph[c], B->rk[c]);
@@ -705,6 +708,8 @@ static int integrate_block(drv_t *D, blk_t *B, int b)
                 dt = dt_new;
                 continue;
             }
+            /* error above 1 accepted at dt_min, rodas3:671-676 */
+            if (!all_accepted && max_err > forced) forced = (float)max_err;
             time_c = time_c + dt;
             dt = fmin(dt_new, time_f - time_c);
             if (time_c < time_f) a->last_accepted_dt[b] = dt_new;
@@ -716,6 +721,7 @@ static int integrate_block(drv_t *D, blk_t *B, int b)
     D->attempts[b] = natt;
     D->substeps[b] = nsub;
     D->failed[b] = failed;
+    D->forced[b] = forced;
     return failed;
 }
 
@@ -755,6 +761,7 @@ int ora_rodas3_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *s
     D->attempts = (int64_t *)calloc(a->nob, sizeof(int64_t));
     D->substeps = (int64_t *)calloc(a->nob, sizeof(int64_t));
     D->failed = (int *)calloc(a->nob, sizeof(int));
+    D->forced = (float *)calloc(a->nob, sizeof(float));
     D->bstart[0] = 0;
     for (int b = 0; b < a->nob; b++) D->bstart[b + 1] = D->bstart[b] + a->block_end[b];
     D->has_sparse = (g_sp_create && g_sp_load_factor && g_sp_solve);
@@ -791,14 +798,20 @@ int ora_rodas3_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *s
         st->substeps = sub;
         st->attempts = att;
         st->failed_blocks = nfail;
+        for (int b = 0; b < a->nob; b++) {
+            if (D->forced[b] > 1.0f) st->forced_blocks++;
+            if (D->forced[b] > st->max_forced_err) st->max_forced_err = D->forced[b];
+        }
         int nz = 0;
         for (int i = 0; i < NS * NS; i++) nz += D->mask[i];
         st->mask_nnz = nz;
     }
     free(g_last_attempts);
     free(g_last_substeps);
+    free(g_last_forced);
     g_last_attempts = D->attempts;
     g_last_substeps = D->substeps;
+    g_last_forced = D->forced;
     g_last_nob = a->nob;
     free(D->failed);
     free(D->bstart);
@@ -816,5 +829,12 @@ int ora_block_counts(int64_t *attempts, int64_t *substeps, int max_nob)
         if (attempts) attempts[b] = g_last_attempts[b];
         if (substeps) substeps[b] = g_last_substeps[b];
     }
+    return g_last_nob;
+}
+
+int ora_block_forced(float *forced, int max_nob)
+{
+    int n = g_last_nob < max_nob ? g_last_nob : max_nob;
+    for (int b = 0; b < n; b++) forced[b] = g_last_forced[b];
     return g_last_nob;
 }

@@ -58,6 +58,8 @@ double ora_dt_new(double dt, double max_err, double dt_min, double dt_max);
 
 /* Per-block counts of the last step: attempts, accepted sub-steps. */
 int ora_block_counts(int64_t *attempts, int64_t *substeps, int max_nob);
+/* Per-block largest error accepted at dt_min (0: none) of the last step. */
+int ora_block_forced(float *forced, int max_nob);
 
 /* Route the LU through an external Sparse1.3 build (the reference's, via
  * oracle/ref_sparse_harness.c).  NULL pointers restore the oracle's own LU.

@@ -50,7 +50,8 @@ class ChemStats(C.Structure):
                 ("attempts", C.c_int64), ("failed_blocks", C.c_int64),
                 ("mask_nnz", C.c_int32), ("kernel_ms", C.c_float),
                 ("cell_substeps", C.c_int64), ("cell_attempts", C.c_int64),
-                ("block1_ms", C.c_float), ("main_ms", C.c_float), ("main_cells", C.c_int64)]
+                ("block1_ms", C.c_float), ("main_ms", C.c_float), ("main_cells", C.c_int64),
+                ("forced_blocks", C.c_int64), ("max_forced_err", C.c_float)]
 
 
 _oracle = None
@@ -267,6 +268,14 @@ def run_oracle(wl, atol, rtol, nthreads=0, split_method=0, n_dyn_chem=1, sc_t=No
         if reference_sparse:
             use_reference_sparse(False)
     return hs, st, mask.reshape(NS, NS), rc
+
+
+def block_forced(nob):
+    f = np.zeros(nob, np.float32)
+    lib = oracle()
+    lib.ora_block_forced.argtypes = [C.POINTER(C.c_float), C.c_int]
+    lib.ora_block_forced(f.ctypes.data_as(C.POINTER(C.c_float)), nob)
+    return f
 
 
 def block_counts(nob):

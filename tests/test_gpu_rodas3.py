@@ -33,6 +33,7 @@ def test_cfg1_device_matches_oracle(seed, tol):
              st.mask_nnz, ost.mask_nnz))
     assert rc == 0 and orc == 0
     assert st.mask_nnz == ost.mask_nnz
+    assert st.forced_blocks == ost.forced_blocks
     assert rep["nfail"] == 0, rep
     np.testing.assert_allclose(dt_gpu, hs.last_dt, rtol=1e-6)
 
