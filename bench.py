@@ -1,25 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark of the CCATT chemistry step on one MI355X.
 
-Workload (N=1): one 120 s chemistry step over a 200x200x40 grid (1.6M cells,
-25 000 blocks of 64), SZA varying smoothly in x (half day, half night), a
-Gaussian NOx/VOC plume (sigma 20 cells, x50), seed 0, rtol 1e-3,
-atol 1e-12 ppb -- the "plain GPU form" workload.  Every step restores the
-same initial state in HBM (a device-to-device copy) and integrates it, so
-each timed step does identical work.
+Workload: a 200x200x40 grid (1.6M cells, 25 000 blocks of 64), dt 120 s,
+SZA varying smoothly in x (half day, half night) and advancing 0.5 deg per
+step, a Gaussian NOx/VOC plume (sigma 20 cells, x50), seed 0, rtol 1e-3,
+atol 1e-12 ppb.  The seeded state is first advanced by --spinup FP64 RODAS3
+steps (untimed; the first of them, the cold step out of the synthetic state,
+is also reported on its own).  From that warm state:
 
-value: cell-steps/s of the GPU path, inputs resident in HBM.  Until the
-surrogate+fallback path exists, the GPU path is the plain FP64 RODAS3 solver
-and the line says so in config.path.
+  leg A (value): --steps consecutive surrogate+fallback steps
+                 (brams_sur_step), bracketed by synchronisations;
+  leg B:         the same --steps steps by the plain FP64 RODAS3 solver, the
+                 accuracy reference (per-species median / p99 relative error
+                 of leg A's states after every step) and the speed reference.
 
-roofline: the dominant kernel (the wave64 RODAS3 kernel over all 64-cell
-blocks) against the FP64 peak; algorithmic FLOP per launch from the generated
-code's exact counts times the device-counted sub-steps and attempts of every
-cell; duration from HIP events on the launch stream.
+Inputs stay resident in HBM; each leg A step also copies the state to a
+snapshot buffer for the accuracy check (a 322 MB device copy, ~0.1 ms).
+--path fp64 times leg B alone as the headline (no surrogate model needed).
 
-cpu_baseline: the oracle (C restatement of the reference step) on a bounded,
-representative sample of the same workload (every 5th column in i and j, all
-40 levels: 64 000 cells), all host cores.
+roofline: the kernel with the largest share of leg A's time, against its
+peak (FP64 for the RODAS3 fallback kernel, BF16 MFMA for the fused MLP);
+the plain FP64 kernel of leg B and the other kernels are listed too.
+Algorithmic FLOP: the generated code's exact counts times the device-counted
+sub-steps and attempts of every cell (RODAS3), 1.356 MFLOP per cell for the
+3-member 4x256 ensemble (MLP).  Durations from HIP events on the launch
+stream.
+
+cpu_baseline: the oracle (C restatement of the reference step) on a bounded
+sample of the warm state (every 5th column in i and j, all 40 levels: 64 000
+cells, one step, step size restarted at dt), all host cores.
 """
 import argparse
 import json
@@ -75,6 +84,10 @@ def sample_workload(wl, stride=5):
     return s
 
 
+MLP_FLOP_PER_CELL = 3 * 2 * (68 * 256 + 3 * 256 * 256 + 256 * 47)
+BF16_PEAK_TFLOPS = 2500.0
+
+
 def cpu_baseline(wl, atol, rtol):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
@@ -85,19 +98,29 @@ def cpu_baseline(wl, atol, rtol):
     dt = time.perf_counter() - t0
     return {"value": sub["grid"].ncell / dt, "unit": "cell-steps/s", "cores": cores,
             "kind": "port",
-            "sample": "oracle (C restatement, -O2) on every 5th column of the workload in i and j: "
-                      "%d cells x 1 step, %.1f s wall, %d attempts" % (sub["grid"].ncell, dt, st.attempts)}
+            "sample": "oracle (C restatement, gcc -O2, pthreads over blocks) on every 5th column of "
+                      "the warm state in i and j: %d cells x 1 step, %.1f s wall, %d attempts"
+                      % (sub["grid"].ncell, dt, st.attempts)}
+
+
+def rodas3_flops(cells, cell_substeps, cell_attempts):
+    f_call, f_sub, f_att = flop_model()
+    return f_call * cells + f_sub * cell_substeps + f_att * cell_attempts
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spinup", type=int, default=6)
     ap.add_argument("--nx", type=int, default=200)
     ap.add_argument("--ny", type=int, default=200)
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--tol", default="baseline", choices=["baseline", "reference"])
+    ap.add_argument("--path", default="surrogate", choices=["surrogate", "fp64"])
+    ap.add_argument("--model", default=os.path.join(ROOT, "brams_amd", "models", "relacs_tuv_surrogate.npz"))
+    ap.add_argument("--spread-tol", type=float, default=None, help="override the model's spread check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -105,67 +128,181 @@ def main():
               "on one grid; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
     from brams_amd import abi
     from brams_amd.chem import DeviceStep
-    from brams_amd.workload import Grid, make_workload, tolerances
+    from brams_amd.surrogate import (NIN, SurrogateModel, SurrogateStep, physical_cells,
+                                     species_errors, trusted_cells)
+    from brams_amd.workload import Grid, jphoto_from_sza, make_workload, tolerances
+
+    def log(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
 
     grid = Grid(args.nx, args.ny, args.nz, dtlt=120.0)
     wl = make_workload(grid, seed=0, sza="split", plume_sigma=20.0, plume_factor=50.0)
     atol, rtol = tolerances(args.tol)
-    step = DeviceStep(wl, atol, rtol, device=0)
-    print("bench: workload %dx%dx%d = %d cells, %d blocks" % (args.nx, args.ny, args.nz, grid.ncell,
-                                                               len(wl["block_end"])), file=sys.stderr, flush=True)
-    for i in range(args.warmup):
-        step.reset()
-        step.step()
-        print("bench: warmup %d done" % i, file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    main_ms, stats = [], None
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step.reset()
-        rc = step.step()
-        s = abi.stats()
-        main_ms.append(s.main_ms)
-        stats = s
-        print("bench: step %d rc=%d kernel %.1f ms (block-1 %.1f ms, main %.1f ms)"
-              % (i, rc, s.kernel_ms, s.block1_ms, s.main_ms), file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / args.steps
-    value = grid.ncell * args.steps / wall
+    K, S = args.steps, args.spinup
+    ref = DeviceStep(wl, atol, rtol, device=0)
+    dev = ref.dev
+    jph = [torch.from_numpy(np.ascontiguousarray(jphoto_from_sza(grid, wl["sza_col"] + 0.5 * t)
+                                                 .ravel(order="F"))).to(dev) for t in range(S + K)]
+    fbuf = torch.empty((grid.ncell, NIN), dtype=torch.float32, device=dev)
+    valid = torch.ones(grid.ncell, dtype=torch.bool, device=dev)
+    log("workload %dx%dx%d = %d cells, %d blocks" % (args.nx, args.ny, args.nz, grid.ncell, len(wl["block_end"])))
 
+    # ---- spin-up (untimed); step 0 is the cold step out of the seeded state
+    cold = None
+    for t in range(S):
+        ref.args.jphoto = jph[t].data_ptr()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ref.step()
+        torch.cuda.synchronize()
+        st = abi.stats()
+        if t == 0:
+            cold = {"ms": 1e3 * (time.perf_counter() - t0), "kernel_ms": st.kernel_ms,
+                    "attempts_per_cell": st.cell_attempts / st.cells,
+                    "substeps_per_cell": st.cell_substeps / st.cells,
+                    "cell_steps_per_s": grid.ncell / (time.perf_counter() - t0)}
+        valid &= trusted_cells(wl["block_end"], dev) & physical_cells(ref, fbuf)
+        log("spin-up step %d: %.1f ms" % (t, st.kernel_ms))
+    warm_sc = ref.sc_p.clone()
+    warm_dt = ref.last_dt.clone()
+
+    sur = None
+    if args.path == "surrogate":
+        model = SurrogateModel.load(args.model)
+        if args.spread_tol is not None:
+            model.spread_tol = args.spread_tol
+        sds = DeviceStep(wl, atol, rtol, device=0)
+        sur = SurrogateStep(model, sds)
+        for i in range(args.warmup):
+            sds.sc_p.copy_(warm_sc)
+            sds.last_dt.copy_(warm_dt)
+            sds.args.jphoto = jph[S].data_ptr()
+            sur.step()
+            log("warmup %d done" % i)
+        sds.sc_p.copy_(warm_sc)
+        sds.last_dt.copy_(warm_dt)
+        snaps = torch.empty((K,) + tuple(sds.sc_p.shape), dtype=sds.sc_p.dtype, device=dev)
+        sstats = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in range(K):
+            sds.args.jphoto = jph[S + t].data_ptr()
+            s = sur.step()
+            snaps[t].copy_(sds.sc_p)
+            sstats.append(s)
+            log("surrogate step %d: %.1f ms (mlp %.2f, fallback %.1f ms, %d fallback cells)"
+                % (t, s.total_ms, s.mlp_ms, s.fallback_ms, s.fallback_cells))
+        torch.cuda.synchronize()
+        wall_a = time.perf_counter() - t0
+
+    # ---- leg B: FP64 rollout from the same warm state
+    ref.sc_p.copy_(warm_sc)
+    ref.last_dt.copy_(warm_dt)
+    wall_b, b_main_ms, b_flops_main, b_kernel_ms = 0.0, 0.0, 0.0, 0.0
+    worst = {"median": 0.0, "p99": 0.0}
+    per_step = []
+    for t in range(K):
+        ref.args.jphoto = jph[S + t].data_ptr()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ref.step()
+        torch.cuda.synchronize()
+        wall_b += time.perf_counter() - t0
+        st = abi.stats()
+        b_main_ms += st.main_ms
+        b_kernel_ms += st.kernel_ms
+        b_flops_main += rodas3_flops(st.cells, st.cell_substeps, st.cell_attempts) * st.main_cells / st.cells
+        valid &= trusted_cells(wl["block_end"], dev) & physical_cells(ref, fbuf)
+        if sur is not None:
+            err = species_errors(snaps[t], ref.sc_p, grid, valid=valid)
+            med = max(err.items(), key=lambda kv: kv[1][0])
+            p99 = max(err.items(), key=lambda kv: kv[1][1])
+            worst["median"] = max(worst["median"], med[1][0])
+            worst["p99"] = max(worst["p99"], p99[1][1])
+            per_step.append([round(med[1][0], 5), med[0], round(p99[1][1], 5), p99[0]])
+        log("fp64 step %d: %.1f ms" % (t, st.kernel_ms))
+    fp64_value = grid.ncell * K / wall_b
+
+    plain = {"kernel": "rodas3_blocks<64,false>", "bound": "fp64", "unit": "TFLOP/s",
+             "avg_ms": b_main_ms / K, "achieved": b_flops_main / (b_main_ms * 1e-3) / 1e12,
+             "peak": FP64_PEAK_TFLOPS}
+    plain["frac"] = plain["achieved"] / plain["peak"]
+    if sur is None:
+        value, wall, path = fp64_value, wall_b, "plain FP64 RODAS3 (GPU), warm rollout"
+        dominant, others = plain, []
+        extra = {}
+    else:
+        value, wall, path = grid.ncell * K / wall_a, wall_a, "surrogate+fallback (GPU), warm rollout"
+        fb_ms = sum(s.fallback_ms for s in sstats)
+        mlp_ms = sum(s.mlp_ms for s in sstats)
+        fb_flops = sum(rodas3_flops(s.fallback_cells, s.fallback_cell_substeps, s.fallback_cell_attempts)
+                       for s in sstats)
+        fallback = {"kernel": "rodas3_blocks<64,false,true>", "bound": "fp64", "unit": "TFLOP/s",
+                    "avg_ms": fb_ms / K, "achieved": fb_flops / max(fb_ms * 1e-3, 1e-9) / 1e12,
+                    "peak": FP64_PEAK_TFLOPS}
+        fallback["frac"] = fallback["achieved"] / fallback["peak"]
+        mlp = {"kernel": "mlp_infer<256,8,2>", "bound": "mfma", "unit": "TFLOP/s",
+               "avg_ms": mlp_ms / K,
+               "achieved": MLP_FLOP_PER_CELL * grid.ncell * K / (mlp_ms * 1e-3) / 1e12,
+               "peak": BF16_PEAK_TFLOPS}
+        mlp["frac"] = mlp["achieved"] / mlp["peak"]
+        if fb_ms >= mlp_ms:
+            dominant, others = fallback, [mlp, plain]
+        else:
+            dominant, others = mlp, [fallback, plain]
+        final = species_errors(snaps[K - 1], ref.sc_p, grid, valid=valid)
+        extra = {
+            "fp64_cell_steps_per_s": fp64_value,
+            "speedup_vs_fp64_rollout": value / fp64_value,
+            "speedup_vs_fp64_cold_step": value / cold["cell_steps_per_s"],
+            "fallback_frac_mean": float(np.mean([s.fallback_cells / s.cells for s in sstats])),
+            "fallback_frac_last": sstats[-1].fallback_cells / sstats[-1].cells,
+            "fallback_reasons_total": dict(zip(["nonfinite", "n_budget", "c_budget", "spread"],
+                                               [int(sum(s.reasons[k] for s in sstats)) for k in range(4)])),
+            "ms_split": {"features": sum(s.features_ms for s in sstats) / K, "mlp": mlp_ms / K,
+                         "apply": sum(s.apply_ms for s in sstats) / K, "fallback": fb_ms / K},
+            "error_final_worst_median": max(v[0] for v in final.values()),
+            "error_final_worst_p99": max(v[1] for v in final.values()),
+            "error_worst_median_over_steps": worst["median"],
+            "error_worst_p99_over_steps": worst["p99"],
+            "error_per_step": per_step,
+            "error_untrusted_fp64_frac": 1.0 - valid.float().mean().item(),
+            "spread_tol": model.spread_tol, "n_tol": model.n_tol, "c_tol": model.c_tol,
+        }
+    roofline = {"bound": "mfma", "achieved": dominant["achieved"], "peak": dominant["peak"],
+                "unit": dominant["unit"], "frac": dominant["frac"], "traffic": None,
+                "kernel": dominant["kernel"], "avg_ms": dominant["avg_ms"],
+                "peak_note": "FP64 vector for the RODAS3 kernels (no MFMA issued; MI355X FP64 vector "
+                             "and matrix peaks are both 78.6 TFLOP/s), dense BF16 MFMA for mlp_infer",
+                "other_kernels": others}
     f_call, f_sub, f_att = flop_model()
-    flops_all = f_call * stats.cells + f_sub * stats.cell_substeps + f_att * stats.cell_attempts
-    frac_main = stats.main_cells / stats.cells
-    flops_main = flops_all * frac_main
-    avg_main_ms = sum(main_ms) / len(main_ms)
-    achieved = flops_main / (avg_main_ms * 1e-3) / 1e12
     line = {
         "metric": "chem cell-steps/s, surrogate+fallback vs FP64 Rosenbrock, at <=1% rel err",
-        "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / K, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if sur is None else "bf16+f64",
         "data": "synthetic (seeded Philox generator, brams_amd/workload.py)",
-        "config": {"workload": "RELACS_TUV RODAS3 step, %dx%dx%d cells (1.6M), dt 120 s, SZA split "
-                               "day/night, NOx/VOC plume sigma 20 x50, seed 0" % (args.nx, args.ny, args.nz),
-                   "path": "plain FP64 RODAS3 (GPU)", "tolerances": args.tol,
-                   "cells": grid.ncell, "blocks": len(wl["block_end"]),
-                   "attempts_per_cell": stats.cell_attempts / stats.cells,
-                   "substeps_per_cell": stats.cell_substeps / stats.cells,
-                   "failed_blocks": stats.failed_blocks, "kernel_ms": stats.kernel_ms,
-                   "block1_ms": stats.block1_ms},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "rodas3_blocks<64,false>", "avg_ms": avg_main_ms,
-                     "flop_per_launch": flops_main,
-                     "flop_model": {"per_cell_call": f_call, "per_substep": f_sub, "per_attempt": f_att},
-                     "peak_note": "FP64 vector (the kernel issues no MFMA; MI355X FP64 vector "
-                                  "and matrix peaks are both 78.6 TFLOP/s)"},
+        "config": dict({"workload": "RELACS_TUV chemistry rollout, %dx%dx%d cells, %d x 120 s steps after "
+                                    "%d FP64 spin-up steps, SZA split day/night +0.5 deg/step, NOx/VOC "
+                                    "plume sigma 20 x50, seed 0" % (args.nx, args.ny, args.nz, K, S),
+                        "path": path, "tolerances": args.tol, "cells": grid.ncell,
+                        "blocks": len(wl["block_end"]), "spinup_steps": S,
+                        "fp64_cold_step": cold,
+                        "flop_model_rodas3": {"per_cell_call": f_call, "per_substep": f_sub,
+                                              "per_attempt": f_att},
+                        "mlp_flop_per_cell": MLP_FLOP_PER_CELL}, **extra),
+        "roofline": roofline,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(wl, atol, rtol)
+        warm_wl = dict(wl)
+        ref.sc_p.copy_(warm_sc)
+        warm_wl["sc_p"] = ref.species_host()
+        warm_wl["jphoto"] = jphoto_from_sza(grid, wl["sza_col"] + 0.5 * S)
+        line["cpu_baseline"] = cpu_baseline(warm_wl, atol, rtol)
     print(json.dumps(line), flush=True)
 
 

@@ -90,6 +90,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ChemError("libbrams_chem.so is not built (run __graft_entry__.build()); "
                             "there is no CPU fallback")
+        # torch ships its own HIP runtime under the soname libamdhip64.so.7.
+        # Loaded first, it also serves this library (one runtime, one HSA
+        # instance); loaded after /opt/rocm's copy, torch would start a
+        # second HIP/HSA runtime that sees no devices.  So torch goes first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         L.brams_chem_init.argtypes = [C.c_int]
         L.brams_chem_rodas3_dyndt.argtypes = [C.POINTER(ChemArgs)]

@@ -17,9 +17,15 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libbrams_chem.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=" + ARCH,
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
+OBJ = os.path.join(PKG, "_build")
+# translation units of libbrams_chem.so; the RODAS3 kernels (minutes of
+# compile) live in rodas3_launch.hip alone
+HIP_UNITS = ["rodas3_launch.hip", "brams_chem.hip", "surrogate.hip"]
+KERNEL_DEPS = ["rodas3_kernel.h", "rodas3_launch.h", "mech_families.h", "generated/relacs_tuv_gen.h",
+               "../../include/brams_chem.h"]
 REF_ROOT = os.environ.get("BRAMS_REFERENCE", "/root/reference")
 
 
@@ -33,10 +39,30 @@ def gen():
     run([sys.executable, os.path.join(ROOT, "oracle", "gen_tables.py")])
 
 
-def build_hip():
-    srcs = [os.path.join(CSRC, "brams_chem.hip"), os.path.join(CSRC, "surrogate.hip")]
+def _headers():
+    out = []
+    for d in (CSRC, os.path.join(CSRC, "generated"), os.path.join(ROOT, "include")):
+        out += [os.path.join(d, f) for f in os.listdir(d) if f.endswith(".h")]
+    return out
+
+
+def build_hip(force=True):
+    """Compile each unit to an object under _build/ and link the library.
+    force=False recompiles only units older than their source or a header."""
+    os.makedirs(OBJ, exist_ok=True)
+    objs = []
+    for unit in HIP_UNITS:
+        src = os.path.join(CSRC, unit)
+        obj = os.path.join(OBJ, unit.replace(".hip", ".o"))
+        deps = [os.path.join(CSRC, h) for h in KERNEL_DEPS] if unit == "rodas3_launch.hip" else _headers()
+        stale = (force or not os.path.exists(obj)
+                 or os.path.getmtime(obj) < max(os.path.getmtime(f) for f in deps + [src]))
+        if stale:
+            run([HIPCC] + HIP_FLAGS + ["-I", CSRC, "-c", "-o", obj + ".tmp", src])
+            os.replace(obj + ".tmp", obj)
+        objs.append(obj)
     tmp = LIB + ".tmp"
-    run([HIPCC] + HIP_FLAGS + ["-I", CSRC, "-o", tmp] + srcs)
+    run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
     os.replace(tmp, LIB)
 
 

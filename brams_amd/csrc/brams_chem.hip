@@ -22,6 +22,7 @@
 #include "../../include/brams_chem.h"
 #include "../../include/brams_surrogate.h"
 #include "rodas3_kernel.h"
+#include "rodas3_launch.h"
 #include "sur_step_kernels.h"
 
 namespace {
@@ -130,23 +131,14 @@ int launch_step(brams::StepParams p, hipStream_t st, int nob) {
   int nbig = 0;
   while (nbig < nob && sz[nbig] > 64) ++nbig;
   HIP_TRY(hipEventRecord(g.ev0, st));
-  if (sz[0] > 64)
-    hipLaunchKernelGGL((brams::rodas3_blocks<128, true>), dim3(1), dim3(128), 0, st, p, 0);
-  else
-    hipLaunchKernelGGL((brams::rodas3_blocks<64, true>), dim3(1), dim3(64), 0, st, p, 0);
-  HIP_TRY(hipGetLastError());
+  HIP_TRY(brams::launch_rodas3(sz[0] > 64 ? 128 : 64, true, 1, p, 0, st));
   HIP_TRY(hipEventRecord(g.evb, st));
-  if (nbig > 1) {
-    hipLaunchKernelGGL((brams::rodas3_blocks<128, false>), dim3(nbig - 1), dim3(128), 0, st, p, 1);
-    HIP_TRY(hipGetLastError());
-  }
+  if (nbig > 1) HIP_TRY(brams::launch_rodas3(128, false, (unsigned)(nbig - 1), p, 1, st));
   const int first64 = nbig > 1 ? nbig : 1;
   HIP_TRY(hipEventRecord(g.evm, st));
   g.main_cells = 0;
   if (nob > first64) {
-    hipLaunchKernelGGL((brams::rodas3_blocks<64, false>), dim3(nob - first64), dim3(64), 0, st, p,
-                       first64);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(brams::launch_rodas3(64, false, (unsigned)(nob - first64), p, first64, st));
     for (int b = first64; b < nob; ++b) g.main_cells += sz[b];
   }
   HIP_TRY(hipEventRecord(g.ev1, st));
@@ -387,12 +379,13 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
   HIP_TRY(hipEventRecord(ev[1], st));
   // 2. ensemble inference on the matrix cores
   brams::SurApplyParams ap{};
+  float* outs[4] = {nullptr, nullptr, nullptr, nullptr};
   for (int k = 0; k < m->nmembers; ++k) {
-    float* o = g.sur_out.as<float>() + (long long)k * nrows * brams::kSurNoutPad;
-    rc = brams_mlp_forward(m->members[k], g.sur_X.p, nrows, o, st);
-    if (rc) return rc;
-    ap.out[k] = o;
+    outs[k] = g.sur_out.as<float>() + (long long)k * nrows * brams::kSurNoutPad;
+    ap.out[k] = outs[k];
   }
+  rc = brams_mlp_forward_ensemble(m->members, m->nmembers, g.sur_X.p, nrows, outs, st);
+  if (rc) return rc;
   HIP_TRY(hipEventRecord(ev[2], st));
   // 3. checks, write-back of accepted cells, ordered list of rejected ones
   ap.nmembers = m->nmembers;
@@ -420,12 +413,17 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
   HIP_TRY(hipEventRecord(ev[3], st));
   HIP_TRY(hipStreamSynchronize(st));
   // 4. RODAS3 on the rejected cells, in blocks of 64 consecutive list entries
-  //    sharing one step (full structural Jacobian, fresh step dtlt*N)
-  long long fb_att = 0;
+  //    sharing one step (full structural Jacobian; the first step is the
+  //    smallest last_accepted_dt of the partition blocks of its cells)
+  long long fb_att = 0, fb_cell_att = 0, fb_cell_sub = 0;
   if (nfb > 0) {
     const int nb = (nfb + 63) / 64;
-    std::vector<double> dt0(nb, (double)(a->dtlt * (float)a->n_dyn_chem));
-    HIP_TRY(hipMemcpyAsync(g.fb_dt.p, dt0.data(), sizeof(double) * nb, hipMemcpyHostToDevice, st));
+    const int* bstart = g.block_start.as<int>();   // the call's partition (ensure_partition)
+    if (!bstart || !p.last_dt || nfb > ncell) return fail(-3, "fallback without a partition");
+    hipLaunchKernelGGL(brams::fallback_dt_hint, dim3((unsigned)nb), dim3(64), 0, st, g.sur_list.as<int>(),
+                       nfb, bstart, (int)a->nob, p.last_dt,
+                       (double)(a->dtlt * (float)a->n_dyn_chem), g.fb_dt.as<double>());
+    HIP_TRY(hipGetLastError());
     unsigned ones[MECH_MASK_WORDS];
     for (auto& w : ones) w = 0xffffffffu;
     HIP_TRY(hipMemcpyAsync(g.mask_all.p, ones, sizeof ones, hipMemcpyHostToDevice, st));
@@ -439,12 +437,17 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
     q.blk_failed = q.blk_attempts + 2 * maxfb;
     q.blk_forced = nullptr;
     q.block_start = nullptr;
-    hipLaunchKernelGGL((brams::rodas3_blocks<64, false>), dim3(nb), dim3(64), 0, st, q, 0);
-    HIP_TRY(hipGetLastError());
-    std::vector<int> att(nb);
+    HIP_TRY(brams::launch_rodas3_list((unsigned)nb, q, st));
+    std::vector<int> att(nb), sub(nb);
     HIP_TRY(hipMemcpyAsync(att.data(), q.blk_attempts, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(sub.data(), q.blk_substeps, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    for (int b = 0; b < nb; ++b) fb_att += att[b];
+    for (int b = 0; b < nb; ++b) {
+      const long long n = std::min(64, nfb - 64 * b);
+      fb_att += att[b];
+      fb_cell_att += n * att[b];
+      fb_cell_sub += n * sub[b];
+    }
   }
   HIP_TRY(hipEventRecord(ev[4], st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -454,6 +457,8 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
     s.fallback_cells = nfb;
     for (int k = 0; k < 4; ++k) s.reasons[k] = reasons[k];
     s.fallback_block_attempts = fb_att;
+    s.fallback_cell_attempts = fb_cell_att;
+    s.fallback_cell_substeps = fb_cell_sub;
     float ms;
     HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); s.features_ms = ms;
     HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[2])); s.mlp_ms = ms;

@@ -202,4 +202,128 @@ __global__ void pack_layer(const float* p, int nout, int nin, int nout_valid, __
   if (WT) WT[(long long)i * nout + o] = b;
 }
 
+// ---------------------------------------------------------------- inference
+// Fused inference of whole networks, all layers of every ensemble member in
+// one launch, activations held in registers.
+//
+// The products run transposed: H'^T[o][cell] = sum_k W[o][k] H^T[k][cell],
+// with the weights as MFMA operand A (32 output features x 16 k) and the
+// activations as operand B (16 k x 32 cells).  A wave owns 32 cells.  Lane
+// l then receives, for output tile t, D rows 32t + 8*(r/4) + 4*(l/32) + r%4
+// of cell l%32 (r = 0..15), and operand B of the next layer wants from the
+// same lane 8 consecutive k of one 16-wide slab.  The output registers
+// acc[8q..8q+7] of tile t are used as they are for slab 2t+q, and the
+// reduction index inside each slab is permuted instead: logical k = 8h + j
+// of lane half h holds the physical feature 16s + perm(h, j) with
+// perm(h, j) = 4h + j (j < 4), 8 + 4h + (j - 4) (j >= 4).  The weights of
+// every layer fed by a hidden layer are packed with that permutation;
+// the first layer reads the feature rows unpermuted.
+//
+// Packed weights ("fragment order"): fragment f = ot * (din/16) + ks of a
+// layer holds 64 lanes x 8 bf16, lane l's 16 bytes being
+// W[32 ot + l%32][16 ks + kperm(l/32, j)], j = 0..7, so every fragment is
+// one coalesced 1 KB load.
+__device__ __forceinline__ int frag_k(int h, int j, bool permute) {
+  return permute ? (j < 4 ? 4 * h + j : 4 + 4 * h + j) : 8 * h + j;
+}
+
+__global__ void pack_frag(const float* __restrict__ p, int nout, int nin, int nout_valid, int permute,
+                          __bf16* __restrict__ F) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nout * nin) return;
+  const int j = (int)(t & 7);
+  const int lane = (int)((t >> 3) & 63);
+  const long long f = t >> 9;
+  const int nks = nin / 16;
+  const int ot = (int)(f / nks), ks = (int)(f % nks);
+  const int o = ot * 32 + (lane & 31);
+  const int k = ks * 16 + frag_k(lane >> 5, j, permute != 0);
+  F[t] = to_bf16(o < nout_valid ? p[(long long)o * nin + k] : 0.f);
+}
+
+constexpr int kInferMaxMembers = 4;
+constexpr int kInferMaxLayers = 9;   // hidden layers + output layer
+
+struct InferArgs {
+  const __bf16* X;            // [M][ldx] bf16 feature rows (ldx = NIN_PAD)
+  long long M;                // rows, a multiple of 32
+  int ldx;
+  int nmembers, nhl;          // members, hidden layers
+  const __bf16* F[kInferMaxMembers][kInferMaxLayers];  // packed weights
+  const float* bias[kInferMaxMembers][kInferMaxLayers];
+  float* out[kInferMaxMembers];                        // [M][NOUT] f32
+};
+
+__device__ __forceinline__ bf16x8_t frag_load(const __bf16* __restrict__ F, int f, int lane) {
+  return *reinterpret_cast<const bf16x8_t*>(F + ((long long)f * 64 + lane) * 8);
+}
+
+// one layer with bias + GELU: NI input slabs -> NO output slabs (NO/2 tiles)
+template <int NI, int NO>
+__device__ __forceinline__ void infer_hidden(const __bf16* __restrict__ F, const float* __restrict__ bias,
+                                             const bf16x8_t (&in)[NI], bf16x8_t (&out)[NO], int lane) {
+  const int h = lane >> 5;
+#pragma unroll
+  for (int ot = 0; ot < NO / 2; ++ot) {
+    f32x16_t acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+#pragma unroll
+    for (int ks = 0; ks < NI; ++ks)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_load(F, ot * NI + ks, lane), in[ks], acc, 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float4 b = *reinterpret_cast<const float4*>(bias + ot * 32 + 8 * q + 4 * h);
+      const float bb[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        out[2 * ot + (q >> 1)][(q & 1) * 4 + e] = to_bf16(gelu_f(acc[4 * q + e] + bb[e]));
+    }
+  }
+}
+
+template <int NH, int NIN_S, int NOUT_T>
+__global__ __launch_bounds__(256) void mlp_infer(InferArgs a) {
+  constexpr int NS = NH / 16;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int h = lane >> 5;
+  const long long cell0 = ((long long)blockIdx.x * 4 + wave) * 32;
+  if (cell0 >= a.M) return;
+  const long long cell = cell0 + (lane & 31);
+  bf16x8_t xin[NIN_S];
+#pragma unroll
+  for (int s = 0; s < NIN_S; ++s)
+    xin[s] = *reinterpret_cast<const bf16x8_t*>(a.X + cell * a.ldx + 16 * s + 8 * h);
+  for (int m = 0; m < a.nmembers; ++m) {
+    bf16x8_t hb[NS];
+    infer_hidden<NIN_S, NS>(a.F[m][0], a.bias[m][0], xin, hb, lane);
+    for (int l = 1; l < a.nhl; ++l) {
+      bf16x8_t hn[NS];
+      infer_hidden<NS, NS>(a.F[m][l], a.bias[m][l], hb, hn, lane);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) hb[s] = hn[s];
+    }
+    const __bf16* F = a.F[m][a.nhl];
+    const float* bias = a.bias[m][a.nhl];
+    float* out = a.out[m] + cell * (32 * NOUT_T);
+#pragma unroll
+    for (int ot = 0; ot < NOUT_T; ++ot) {
+      f32x16_t acc;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+#pragma unroll
+      for (int ks = 0; ks < NS; ++ks)
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_load(F, ot * NS + ks, lane), hb[ks], acc, 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int o = ot * 32 + 8 * q + 4 * h;
+        const float4 b = *reinterpret_cast<const float4*>(bias + o);
+        *reinterpret_cast<float4*>(out + o) =
+            make_float4(acc[4 * q] + b.x, acc[4 * q + 1] + b.y, acc[4 * q + 2] + b.z, acc[4 * q + 3] + b.w);
+      }
+    }
+  }
+}
+
 }  // namespace brams

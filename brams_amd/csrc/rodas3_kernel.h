@@ -83,11 +83,13 @@ __device__ __forceinline__ void block_reduce(double& errmax, int& rej, int& fail
 // DYN: block 1 of the call.  Its sparsity mask is recomputed from cell 1 on
 // every attempt (rodas3:470-482) and the last one is published for the other
 // blocks.  Otherwise the mask is read from p.mask.
-template <int WG, bool DYN>
+// LIST: the blocks are consecutive WG-sized runs of p.cell_list (fallback of
+// the surrogate step) instead of the AllocIndex partition.
+template <int WG, bool DYN, bool LIST = false>
 __global__ __launch_bounds__(WG) void rodas3_blocks(StepParams p, int block0) {
   const int b = block0 + blockIdx.x;
   int start, ncell;
-  if (p.cell_list) {                                // fallback blocks of listed cells
+  if constexpr (LIST) {                             // fallback blocks of listed cells
     start = b * WG;
     ncell = min(WG, p.list_n - start);
   } else {
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(WG) void rodas3_blocks(StepParams p, int block0) {
   }
   const int c = threadIdx.x;
   const bool act = c < ncell;
-  const long long g = p.cell_list ? (long long)p.cell_list[start + (act ? c : 0)]
+  const long long g = LIST ? (long long)p.cell_list[start + (act ? c : 0)]
                                   : (long long)(start + (act ? c : 0));  // idle lanes shadow cell 1
 
   // cell position, j -> i -> k order (chem_spack_utils.f90:144-160)

@@ -282,4 +282,28 @@ __global__ void sur_apply(StepParams p, SurApplyParams a, long long ncell) {
   }
 }
 
+// Initial step of each fallback block (64 consecutive list entries): the
+// smallest step carried by the partition blocks its cells belong to
+// (last_accepted_dt, rodas3:686), found by bisection of block_start.
+__global__ __launch_bounds__(64) void fallback_dt_hint(const int* __restrict__ list, int nfb,
+                                                       const int* __restrict__ block_start, int nob,
+                                                       const double* __restrict__ last_dt, double dt_max,
+                                                       double* __restrict__ fb_dt) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  double d = dt_max;
+  if (i < nfb) {
+    const int g = list[i];
+    int lo = 0, hi = nob - 1;                 // largest b with block_start[b] <= g
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (block_start[mid] <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    d = fmin(d, last_dt[lo]);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) d = fmin(d, __shfl_xor(d, off, 64));
+  if (threadIdx.x == 0) fb_dt[blockIdx.x] = d;
+}
+
 }  // namespace brams

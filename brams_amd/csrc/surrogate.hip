@@ -1,10 +1,11 @@
 // Host side of the BF16 MLP surrogate (include/brams_surrogate.h).
 //
 // One brams_mlp owns FP32 master parameters, Adam moments, bf16 copies of
-// every weight matrix (W[out][in] for the forward product and WT[in][out]
-// for the input gradient) and the activation buffers of one training batch.
-// Every product runs on gemm_bf16 (mlp_kernels.h); nothing here falls back
-// to a library GEMM or to the CPU.
+// every weight matrix (W[out][in] for the forward product, WT[in][out] for
+// the input gradient, and the fragment-ordered copy read by the fused
+// inference kernel) and the activation buffers of one training batch.
+// Training products run on gemm_bf16, inference on mlp_infer (both in
+// mlp_kernels.h); nothing here falls back to a library GEMM or to the CPU.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -64,6 +65,8 @@ struct brams_mlp {
   long long np = 0;
   float *p = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;
   std::vector<__bf16*> W, WT;
+  std::vector<__bf16*> Fr;                      // fragment-ordered weights for mlp_infer
+  bool frag_dirty = true;                       // Fr older than p
   // training activations
   std::vector<float*> Z;                        // pre-activations [maxb][nh]
   std::vector<__bf16*> A;                       // activations [maxb][nh]
@@ -73,9 +76,6 @@ struct brams_mlp {
   __bf16 *dout_b = nullptr, *Xb = nullptr;
   long long* idx = nullptr;
   long long idx_n = 0;
-  // inference ping-pong buffers
-  __bf16 *inf0 = nullptr, *inf1 = nullptr;
-  long long inf_n = 0;
   long long step = 0;
 };
 
@@ -88,11 +88,31 @@ int pack_all(brams_mlp* m, hipStream_t st) {
                        m->p + m->woff[l], m->dout[l], m->din[l], m->dvalid[l], m->W[l], m->WT[l]);
     SUR_TRY(hipGetLastError());
   }
+  m->frag_dirty = true;
   return 0;
 }
 
-// forward through all layers; training keeps Z/A for the backward pass
-int forward(brams_mlp* m, const __bf16* X, long long M, float* out, bool training, hipStream_t st) {
+int pack_frags(brams_mlp* m, hipStream_t st) {
+  for (int l = 0; l < m->nlay; ++l) {
+    const long long n = (long long)m->dout[l] * m->din[l];
+    hipLaunchKernelGGL(brams::pack_frag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       m->p + m->woff[l], m->dout[l], m->din[l], m->dvalid[l], l > 0 ? 1 : 0, m->Fr[l]);
+    SUR_TRY(hipGetLastError());
+  }
+  m->frag_dirty = false;
+  return 0;
+}
+
+template <int NH>
+hipError_t launch_infer(const brams::InferArgs& a, hipStream_t st) {
+  const unsigned grid = (unsigned)((a.M + 127) / 128);
+  hipLaunchKernelGGL((brams::mlp_infer<NH, BRAMS_SUR_NIN_PAD / 16, BRAMS_SUR_NOUT_PAD / 32>), dim3(grid),
+                     dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// training forward through all layers, keeping Z/A for the backward pass
+int forward(brams_mlp* m, const __bf16* X, long long M, float* out, hipStream_t st) {
   const __bf16* in = X;
   long long ldin = BRAMS_SUR_NIN_PAD;
   for (int l = 0; l < m->nlay; ++l) {
@@ -105,17 +125,11 @@ int forward(brams_mlp* m, const __bf16* X, long long M, float* out, bool trainin
     if (l == m->nlay - 1) {
       g.O32 = out; g.ldo32 = m->dout[l];
       SUR_TRY((gemm<brams::LAY_N, brams::LAY_N, brams::EPI_BIAS>(g, st)));
-    } else if (training) {
+    } else {
       g.Z = m->Z[l]; g.ldz = m->nh;
       g.O = m->A[l]; g.ldo = m->nh;
       SUR_TRY((gemm<brams::LAY_N, brams::LAY_N, brams::EPI_BIAS_GELU>(g, st)));
       in = m->A[l];
-      ldin = m->nh;
-    } else {
-      __bf16* o = (l % 2 == 0) ? m->inf0 : m->inf1;
-      g.O = o; g.ldo = m->nh;
-      SUR_TRY((gemm<brams::LAY_N, brams::LAY_N, brams::EPI_BIAS_GELU_INF>(g, st)));
-      in = o;
       ldin = m->nh;
     }
   }
@@ -198,8 +212,10 @@ extern "C" {
 
 int brams_mlp_create(int nhidden, int nlayers, int max_batch, uint64_t seed, brams_mlp** out) {
   if (!out) return brams_chem_set_error(-2, "null output");
-  if (nhidden < 64 || nhidden % 64 || nlayers < 1 || max_batch < 64 || max_batch % 64)
-    return brams_chem_set_error(-2, "nhidden and max_batch must be multiples of 64, nlayers >= 1");
+  if (nhidden < 64 || nhidden > 256 || nhidden % 64 || nlayers < 1 ||
+      nlayers + 1 > brams::kInferMaxLayers || max_batch < 64 || max_batch % 64)
+    return brams_chem_set_error(-2, "nhidden must be 64, 128, 192 or 256, nlayers 1..8, "
+                                    "max_batch a multiple of 64");
   brams_mlp* m = new brams_mlp();
   m->nh = nhidden; m->nl = nlayers; m->maxb = max_batch; m->nlay = nlayers + 1;
   for (int l = 0; l < m->nlay; ++l) {
@@ -241,9 +257,11 @@ int brams_mlp_create(int nhidden, int nlayers, int max_batch, uint64_t seed, bra
   A_(hipMemcpy(m->v, zero.data(), sizeof(float) * m->np, hipMemcpyHostToDevice));
   m->W.resize(m->nlay);
   m->WT.resize(m->nlay);
+  m->Fr.resize(m->nlay);
   for (int l = 0; l < m->nlay; ++l) {
     A_(dalloc(&m->W[l], (size_t)m->dout[l] * m->din[l]));
     A_(dalloc(&m->WT[l], (size_t)m->dout[l] * m->din[l]));
+    A_(dalloc(&m->Fr[l], (size_t)m->dout[l] * m->din[l]));
   }
   m->Z.resize(nlayers); m->A.resize(nlayers); m->dZ.resize(nlayers); m->dZb.resize(nlayers);
   for (int l = 0; l < nlayers; ++l) {
@@ -279,11 +297,12 @@ int brams_mlp_create(int nhidden, int nlayers, int max_batch, uint64_t seed, bra
 void brams_mlp_destroy(brams_mlp* m) {
   if (!m) return;
   void* ps[] = {m->p, m->m, m->v, m->g, m->out, m->dout_f, m->dout_b, m->Xb, m->Yb, m->loss,
-                m->idx, m->inf0, m->inf1};
+                m->idx};
   for (void* p : ps)
     if (p) hipFree(p);
   for (auto* p : m->W) if (p) hipFree(p);
   for (auto* p : m->WT) if (p) hipFree(p);
+  for (auto* p : m->Fr) if (p) hipFree(p);
   for (auto* p : m->Z) if (p) hipFree(p);
   for (auto* p : m->A) if (p) hipFree(p);
   for (auto* p : m->dZ) if (p) hipFree(p);
@@ -308,21 +327,46 @@ int brams_mlp_set_params(brams_mlp* m, const float* host, int64_t n) {
   return 0;
 }
 
-int brams_mlp_forward(brams_mlp* m, const void* X, int64_t M, float* out, void* stream) {
-  if (!m || !X || !out || M <= 0 || M % 64) return brams_chem_set_error(-2, "bad forward arguments");
+int brams_mlp_forward_ensemble(brams_mlp* const* members, int n, const void* X, int64_t M,
+                               float* const* out, void* stream) {
+  if (!members || n < 1 || n > brams::kInferMaxMembers || !X || !out || M <= 0 || M % 64)
+    return brams_chem_set_error(-2, "bad forward arguments");
   hipStream_t st = (hipStream_t)stream;
-  if (m->inf_n < M) {
-    if (m->inf0) hipFree(m->inf0);
-    if (m->inf1) hipFree(m->inf1);
-    m->inf0 = m->inf1 = nullptr;
-    m->inf_n = 0;
-    SUR_TRY(dalloc(&m->inf0, (size_t)M * m->nh));
-    SUR_TRY(dalloc(&m->inf1, (size_t)M * m->nh));
-    m->inf_n = M;
+  const brams_mlp* m0 = members[0];
+  brams::InferArgs a{};
+  a.X = (const __bf16*)X;
+  a.M = M;
+  a.ldx = BRAMS_SUR_NIN_PAD;
+  a.nmembers = n;
+  a.nhl = m0 ? m0->nl : 0;
+  for (int k = 0; k < n; ++k) {
+    brams_mlp* m = members[k];
+    if (!m || !out[k] || m->nh != m0->nh || m->nl != m0->nl)
+      return brams_chem_set_error(-2, "ensemble members differ in shape");
+    if (m->frag_dirty) {
+      int rc = pack_frags(m, st);
+      if (rc) return rc;
+    }
+    for (int l = 0; l < m->nlay; ++l) {
+      a.F[k][l] = m->Fr[l];
+      a.bias[k][l] = m->p + m->boff[l];
+    }
+    a.out[k] = out[k];
   }
-  int rc = forward(m, (const __bf16*)X, M, out, false, st);
+  switch (m0->nh) {
+    case 64: SUR_TRY(launch_infer<64>(a, st)); break;
+    case 128: SUR_TRY(launch_infer<128>(a, st)); break;
+    case 192: SUR_TRY(launch_infer<192>(a, st)); break;
+    case 256: SUR_TRY(launch_infer<256>(a, st)); break;
+    default: return brams_chem_set_error(-2, "unsupported hidden width");
+  }
+  return 0;
+}
+
+int brams_mlp_forward(brams_mlp* m, const void* X, int64_t M, float* out, void* stream) {
+  int rc = brams_mlp_forward_ensemble(&m, 1, X, M, &out, stream);
   if (rc) return rc;
-  SUR_TRY(hipStreamSynchronize(st));
+  SUR_TRY(hipStreamSynchronize((hipStream_t)stream));
   return 0;
 }
 
@@ -330,7 +374,7 @@ int brams_mlp_gradients(brams_mlp* m, const void* X, const float* Y, int B, floa
   if (!m || !X || !Y || B <= 0 || B % 64 || B > m->maxb)
     return brams_chem_set_error(-2, "bad gradient arguments");
   SUR_TRY(fill(m->loss, 1, 0.f, nullptr));
-  int rc = forward(m, (const __bf16*)X, B, m->out, true, nullptr);
+  int rc = forward(m, (const __bf16*)X, B, m->out, nullptr);
   if (!rc) rc = backward(m, (const __bf16*)X, Y, B, nullptr);
   if (rc) return rc;
   SUR_TRY(hipDeviceSynchronize());
@@ -364,7 +408,7 @@ int brams_mlp_train(brams_mlp* m, const void* X, const float* Y, int64_t N, int 
                          BRAMS_SUR_NIN_PAD, Y, BRAMS_SUR_NOUT_PAD, m->idx + b * batch, batch, m->Xb,
                          m->Yb);
       SUR_TRY(hipGetLastError());
-      int rc = forward(m, m->Xb, batch, m->out, true, nullptr);
+      int rc = forward(m, m->Xb, batch, m->out, nullptr);
       if (!rc) rc = backward(m, m->Xb, m->Yb, batch, nullptr);
       const float lr = (float)(0.5 * lr0 * (1.0 + std::cos(M_PI * (double)t / (double)total)));
       if (!rc) rc = adam(m, lr, nullptr);

@@ -24,7 +24,8 @@ from . import abi
 from .mechanism import load_mechanism
 from .mlp import MLP, NIN, NIN_PAD, NOUT, NOUT_PAD
 
-FLOOR_C = 1.0e-3          # molec/cm^3 floor of the log10 features / targets
+FLOOR_C = 1.0e-5          # molec/cm^3 floor of the log10 features / targets (below the
+                          # 1e-12 ppb error floor, 6e-3 molec/cm^3 and up)
 # Blocks whose step accepted an error above this at dt_min (the reference's
 # forced acceptance, chem_spack_rodas3_dyndt.f90:671-676) or that failed are
 # not trustworthy FP64 results: their cells are left out of datasets and
@@ -63,7 +64,8 @@ class SurModel(C.Structure):
 class SurStats(C.Structure):
     """brams_sur_stats"""
     _fields_ = [("cells", C.c_int64), ("fallback_cells", C.c_int64), ("reasons", C.c_int32 * 4),
-                ("fallback_block_attempts", C.c_int64), ("features_ms", C.c_float),
+                ("fallback_block_attempts", C.c_int64), ("fallback_cell_attempts", C.c_int64),
+                ("fallback_cell_substeps", C.c_int64), ("features_ms", C.c_float),
                 ("mlp_ms", C.c_float), ("apply_ms", C.c_float), ("fallback_ms", C.c_float),
                 ("total_ms", C.c_float)]
 
@@ -96,13 +98,14 @@ def atom_vectors():
 
 
 # ------------------------------------------------------------------ dataset
-def make_dataset(grid, seeds, steps, fraction, atol, rtol, sza_step=0.5, log=print, rng_seed=0):
+def make_dataset(grid, seeds, steps, fraction, atol, rtol, sza_step=0.5, log=print, rng_seed=0, skip=0):
     """Raw (features, targets) of FP64 RODAS3 rollouts on the GPU.
 
     For every seed a workload on `grid` is integrated for `steps` chemistry
     steps, SZA advancing `sza_step` degrees per step; before each step the
     features of every cell are taken, after it the log10 increments, and a
-    random `fraction` of the cells is kept.  Returns device tensors
+    random `fraction` of the cells is kept (none of the first `skip` steps,
+    the transient out of the synthetic initial state).  Returns device tensors
     (feat [N][68] f32, targ [N][47] f32)."""
     import torch
     from .chem import DeviceStep
@@ -130,10 +133,11 @@ def make_dataset(grid, seeds, steps, fraction, atol, rtol, sza_step=0.5, log=pri
             valid &= trusted_cells(wl["block_end"], st.dev)
             valid &= torch.isfinite(fbuf).all(1) & torch.isfinite(tbuf).all(1)
             valid &= ((fbuf[:, :NOUT] < LOG10_C_MAX) & (fbuf[:, :NOUT] + tbuf < LOG10_C_MAX)).all(1)
-            keep = torch.randperm(ncell, generator=gen)[:int(fraction * ncell)].to(st.dev)
-            keep = keep[valid[keep]]
-            feats.append(fbuf.index_select(0, keep))
-            targs.append(tbuf.index_select(0, keep))
+            if t >= skip:
+                keep = torch.randperm(ncell, generator=gen)[:int(fraction * ncell)].to(st.dev)
+                keep = keep[valid[keep]]
+                feats.append(fbuf.index_select(0, keep))
+                targs.append(tbuf.index_select(0, keep))
             del jph, before
         log("dataset: seed %d, %d steps, %.1f s, %.5f of the cells left out"
             % (seed, steps, time.time() - t0, 1.0 - valid.float().mean().item()))

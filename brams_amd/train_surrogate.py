@@ -52,8 +52,9 @@ def calibrate_budgets(model, feat, targ, nsample=400000):
     return out
 
 
-def rollout(model, grid, steps, atol, rtol, seed=0, sza_step=0.5):
-    """Surrogate+fallback and FP64 rollouts from the same initial state."""
+def rollout(model, grid, steps, atol, rtol, seed=0, sza_step=0.5, spinup=0):
+    """Surrogate+fallback and FP64 rollouts from the same initial state: the
+    seeded workload after `spinup` FP64 steps (untimed, shared by both)."""
     import torch
     from brams_amd.chem import DeviceStep
     from brams_amd.surrogate import NIN, SurrogateStep, physical_cells, species_errors, trusted_cells
@@ -63,12 +64,20 @@ def rollout(model, grid, steps, atol, rtol, seed=0, sza_step=0.5):
     sur_state = DeviceStep(wl, atol, rtol)
     sur = SurrogateStep(model, sur_state)
     jph = [torch.from_numpy(np.ascontiguousarray(jphoto_from_sza(grid, wl["sza_col"] + sza_step * t)
-                                                 .ravel(order="F"))).to(ref.dev) for t in range(steps)]
+                                                 .ravel(order="F"))).to(ref.dev) for t in range(spinup + steps)]
+    for t in range(spinup):
+        ref.args.jphoto = jph[t].data_ptr()
+        ref.step()
+    sur_state.sc_p.copy_(ref.sc_p)
+    sur_state.last_dt.copy_(ref.last_dt)
+    jph = jph[spinup:]
     torch.cuda.synchronize()
-    report = {"steps": []}
+    report = {"steps": [], "spinup": spinup}
     t_ref = t_sur = 0.0
     valid = torch.ones(grid.ncell, dtype=torch.bool, device=ref.dev)
     fbuf = torch.empty((grid.ncell, NIN), dtype=torch.float32, device=ref.dev)
+    if spinup:
+        valid &= trusted_cells(wl["block_end"], ref.dev) & physical_cells(ref, fbuf)
     for t in range(steps):
         ref.args.jphoto = jph[t].data_ptr()
         sur_state.args.jphoto = jph[t].data_ptr()
@@ -116,6 +125,8 @@ def main():
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=os.path.join(ROOT, "brams_amd", "models", "relacs_tuv_surrogate.npz"))
+    ap.add_argument("--skip", type=int, default=2)
+    ap.add_argument("--spinup", type=int, default=6)
     ap.add_argument("--eval-steps", type=int, default=24)
     ap.add_argument("--report", default=None)
     args = ap.parse_args()
@@ -127,13 +138,13 @@ def main():
     atol, rtol = tolerances("baseline")
     seeds = [int(s) for s in args.seeds.split(",") if s]
     t0 = time.time()
-    feat, targ = make_dataset(grid, seeds, args.steps, args.fraction, atol, rtol, log=log)
+    feat, targ = make_dataset(grid, seeds, args.steps, args.fraction, atol, rtol, log=log, skip=args.skip)
     log("dataset: %d cell-steps in %.1f s" % (feat.shape[0], time.time() - t0))
     model = train_ensemble(feat, targ, nmembers=args.members, epochs=args.epochs, batch=args.batch,
                            lr0=args.lr, log=log)
     ntol, ctol = calibrate_budgets(model, feat, targ)
     log("checks: N budget tol %.3g, C budget tol %.3g, spread tol %.3g" % (ntol, ctol, model.spread_tol))
-    model.meta.update({"seeds": seeds, "steps": args.steps, "fraction": args.fraction,
+    model.meta.update({"seeds": seeds, "steps": args.steps, "skip": args.skip, "fraction": args.fraction,
                        "grid": [args.nx, args.ny, args.nz]})
     del feat, targ
     torch.cuda.empty_cache()
@@ -141,7 +152,7 @@ def main():
     model.save(args.out)
     log("saved %s" % args.out)
     if args.eval_steps > 0:
-        rep = rollout(model, grid, args.eval_steps, atol, rtol)
+        rep = rollout(model, grid, args.eval_steps, atol, rtol, spinup=args.spinup)
         rep["model"] = model.meta
         log("rollout: fp64 %.4g cell-steps/s, surrogate %.4g cell-steps/s, speedup %.2f"
             % (rep["fp64_cell_steps_per_s"], rep["surrogate_cell_steps_per_s"], rep["speedup"]))

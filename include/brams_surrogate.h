@@ -43,8 +43,15 @@ int brams_mlp_get_params(const brams_mlp *m, float *host, int64_t n);
 int brams_mlp_set_params(brams_mlp *m, const float *host, int64_t n);
 
 /* Forward pass: X [M][BRAMS_SUR_NIN_PAD] bf16 -> out [M][BRAMS_SUR_NOUT_PAD]
- * f32, M a multiple of 64 (any size; inference buffers are allocated here). */
+ * f32, M a multiple of 64; synchronous.  Runs the fused inference kernel
+ * (all layers in one launch, activations in registers). */
 int brams_mlp_forward(brams_mlp *m, const void *X, int64_t M, float *out, void *stream);
+
+/* Forward pass of n <= 4 networks of one shape over the same rows in one
+ * launch: out[k] [M][BRAMS_SUR_NOUT_PAD] f32 for member k.  Stream-ordered
+ * (returns before the kernel ends). */
+int brams_mlp_forward_ensemble(brams_mlp *const *members, int n, const void *X, int64_t M,
+                               float *const *out, void *stream);
 
 /* Gradient of the mean squared error over the first BRAMS_SUR_NOUT outputs
  * of one batch (B <= max_batch): grads (host, num_params floats) receives
@@ -81,6 +88,8 @@ typedef struct brams_sur_stats {
     int64_t fallback_cells;          /* cells integrated by RODAS3 */
     int32_t reasons[4];              /* non-finite, N budget, C budget, spread */
     int64_t fallback_block_attempts; /* RODAS3 attempts summed over fallback blocks */
+    int64_t fallback_cell_attempts;  /* attempts x cells, summed over fallback blocks */
+    int64_t fallback_cell_substeps;  /* accepted sub-steps x cells, likewise */
     float features_ms, mlp_ms, apply_ms, fallback_ms, total_ms;
 } brams_sur_stats;
 
