@@ -245,7 +245,7 @@ def main():
                     "avg_ms": fb_ms / K, "achieved": fb_flops / max(fb_ms * 1e-3, 1e-9) / 1e12,
                     "peak": FP64_PEAK_TFLOPS}
         fallback["frac"] = fallback["achieved"] / fallback["peak"]
-        mlp = {"kernel": "mlp_infer<256,8,2>", "bound": "mfma", "unit": "TFLOP/s",
+        mlp = {"kernel": "mlp_infer<256,4,8,2>", "bound": "mfma", "unit": "TFLOP/s",
                "avg_ms": mlp_ms / K,
                "achieved": MLP_FLOP_PER_CELL * grid.ncell * K / (mlp_ms * 1e-3) / 1e12,
                "peak": BF16_PEAK_TFLOPS}

@@ -28,16 +28,36 @@ namespace brams {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16_t;
+typedef __attribute__((__vector_size__(4 * sizeof(unsigned)))) unsigned u32x4_t;
 
 enum { LAY_N = 0, LAY_T = 1 };
 enum { EPI_BIAS_GELU = 0, EPI_BIAS = 1, EPI_GELU_BWD = 2, EPI_ACCUM = 3, EPI_BIAS_GELU_INF = 4 };
 
+// GELU(x) = x Phi(x) = 0.5 x (1 + erf(x / sqrt 2)), with erf from
+// Abramowitz & Stegun, Handbook of Mathematical Functions (1964), 7.1.26
+// (Hastings' rational approximation): for z >= 0
+//   erf(z) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-z^2),
+//   t = 1 / (1 + p z),  |error| <= 1.5e-7,
+// and erf(-z) = -erf(z).  Branch-free: about 15 VALU operations against
+// some 35 with data-dependent branches for the library erff, which matters
+// because every hidden activation passes through it.  Training and
+// inference use the same function.
+__device__ __forceinline__ float erfc_tail(float z) {   // 1 - erf(z), z >= 0
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, z, 1.0f));
+  float q = fmaf(1.061405429f, t, -1.453152027f);
+  q = fmaf(q, t, 1.421413741f);
+  q = fmaf(q, t, -0.284496736f);
+  q = fmaf(q, t, 0.254829592f);
+  return q * t * __builtin_amdgcn_exp2f(-1.4426950408889634f * z * z);
+}
 __device__ __forceinline__ float gelu_f(float x) {
-  return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
+  const float h = 0.5f * x * erfc_tail(fabsf(x) * 0.70710678118654752f);
+  return x >= 0.0f ? x - h : h;
 }
 __device__ __forceinline__ float gelu_grad_f(float x) {
-  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * __expf(-0.5f * x * x);
+  const float tail = 0.5f * erfc_tail(fabsf(x) * 0.70710678118654752f);
+  const float cdf = x >= 0.0f ? 1.0f - tail : tail;
+  const float pdf = 0.39894228040143268f * __builtin_amdgcn_exp2f(-0.72134752044448170f * x * x);
   return cdf + x * pdf;
 }
 
@@ -254,73 +274,143 @@ struct InferArgs {
   float* out[kInferMaxMembers];                        // [M][NOUT] f32
 };
 
-__device__ __forceinline__ bf16x8_t frag_load(const __bf16* __restrict__ F, int f, int lane) {
-  return *reinterpret_cast<const bf16x8_t*>(F + ((long long)f * 64 + lane) * 8);
+// Weight tiles are staged through LDS: the 256 threads of a workgroup load
+// tile t+1 (NI fragments, NI KB) from global memory into registers before
+// the MFMAs of tile t and store it into the LDS tile buffer after them,
+// between two barriers.  Each fragment crosses L2 once per workgroup (four
+// waves, 128 cells) and reaches a wave as one 16-byte LDS load per lane.
+// A layer's outputs go to a per-wave LDS buffer in the operand-B lane order
+// and are read back as the next layer's input, so one activation array (not
+// two) lives in registers; 80 KB of LDS per workgroup leaves room for two
+// workgroups per CU.
+// A fetch always moves a whole tile buffer (FMAX fragments) so that the
+// prefetch registers stay registers; a tile of fewer fragments is followed
+// by the next one or by the kInferPad bytes of padding behind every packed
+// layer, and the MFMAs read only its own fragments.
+constexpr int kInferPadElems = 16 * 512;   // bf16 elements behind each packed layer
+
+template <int NL>
+__device__ __forceinline__ void tile_fetch(const __bf16* __restrict__ src, int tid, u32x4_t (&r)[NL]) {
+  const u32x4_t* s = reinterpret_cast<const u32x4_t*>(src);
+#pragma unroll
+  for (int i = 0; i < NL; ++i) r[i] = s[i * 256 + tid];
 }
 
-// one layer with bias + GELU: NI input slabs -> NO output slabs (NO/2 tiles)
-template <int NI, int NO>
-__device__ __forceinline__ void infer_hidden(const __bf16* __restrict__ F, const float* __restrict__ bias,
-                                             const bf16x8_t (&in)[NI], bf16x8_t (&out)[NO], int lane) {
-  const int h = lane >> 5;
+template <int NL>
+__device__ __forceinline__ void tile_store(u32x4_t* dst, int tid, const u32x4_t (&r)[NL]) {
 #pragma unroll
-  for (int ot = 0; ot < NO / 2; ++ot) {
-    f32x16_t acc;
+  for (int i = 0; i < NL; ++i) dst[i * 256 + tid] = r[i];
+}
+
+// the tile after (m, l, t) in the order members -> layers -> output tiles
+// (after the last one: the first tile again, fetched and never used)
+template <int NH, int NHL, int NIN_S, int NOUT_T>
+__device__ __forceinline__ const __bf16* next_tile(const InferArgs& a, int m, int l, int t) {
+  constexpr int NT_H = NH / 32, NS = NH / 16;
+  const int ntl = (l == NHL) ? NOUT_T : NT_H;
+  if (t + 1 < ntl) {
+    ++t;
+  } else if (l < NHL) {
+    ++l;
+    t = 0;
+  } else if (m + 1 < a.nmembers) {
+    ++m;
+    l = 0;
+    t = 0;
+  } else {
+    return a.F[0][0];
+  }
+  return a.F[m][l] + (long long)t * ((l == 0) ? NIN_S : NS) * 512;
+}
+
+template <int NI, int N>
+__device__ __forceinline__ f32x16_t tile_mfma(const u32x4_t* w, const bf16x8_t (&in)[N], int lane) {
+  static_assert(NI <= N, "operand slabs");
+  f32x16_t acc;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
 #pragma unroll
-    for (int ks = 0; ks < NI; ++ks)
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_load(F, ot * NI + ks, lane), in[ks], acc, 0, 0, 0);
+  for (int ks = 0; ks < NI; ++ks) {
+    const bf16x8_t f = __builtin_bit_cast(bf16x8_t, w[ks * 64 + lane]);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f, in[ks], acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// bias + GELU of output tile t -> slabs 2t, 2t+1 of the wave's LDS buffer
+__device__ __forceinline__ void tile_gelu_lds(const f32x16_t& acc, const float* __restrict__ bias, int t, int h,
+                                              int lane, u32x4_t* act) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float4 b = *reinterpret_cast<const float4*>(bias + ot * 32 + 8 * q + 4 * h);
-      const float bb[4] = {b.x, b.y, b.z, b.w};
+  for (int half = 0; half < 2; ++half) {
+    bf16x8_t o;
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        out[2 * ot + (q >> 1)][(q & 1) * 4 + e] = to_bf16(gelu_f(acc[4 * q + e] + bb[e]));
+    for (int qq = 0; qq < 2; ++qq) {
+      const int q = 2 * half + qq;
+      const float4 b = *reinterpret_cast<const float4*>(bias + t * 32 + 8 * q + 4 * h);
+      o[4 * qq + 0] = to_bf16(gelu_f(acc[4 * q + 0] + b.x));
+      o[4 * qq + 1] = to_bf16(gelu_f(acc[4 * q + 1] + b.y));
+      o[4 * qq + 2] = to_bf16(gelu_f(acc[4 * q + 2] + b.z));
+      o[4 * qq + 3] = to_bf16(gelu_f(acc[4 * q + 3] + b.w));
     }
+    act[(2 * t + half) * 64 + lane] = __builtin_bit_cast(u32x4_t, o);
   }
 }
 
-template <int NH, int NIN_S, int NOUT_T>
+template <int NH, int NHL, int NIN_S, int NOUT_T>
 __global__ __launch_bounds__(256) void mlp_infer(InferArgs a) {
   constexpr int NS = NH / 16;
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
+  constexpr int NT_H = NH / 32;
+  constexpr int FMAX = NS > NIN_S ? NS : NIN_S;
+  static_assert(FMAX <= 16 && FMAX % 4 == 0, "a tile is staged by 256 threads x FMAX/4 x 16 bytes");
+  static_assert(FMAX * 512 <= kInferPadElems, "padding behind each packed layer");
+  __shared__ u32x4_t wbuf[FMAX * 64];
+  __shared__ u32x4_t act_all[4][NS * 64];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
   const int h = lane >> 5;
+  u32x4_t* act = act_all[wave];
   const long long cell0 = ((long long)blockIdx.x * 4 + wave) * 32;
-  if (cell0 >= a.M) return;
-  const long long cell = cell0 + (lane & 31);
-  bf16x8_t xin[NIN_S];
-#pragma unroll
-  for (int s = 0; s < NIN_S; ++s)
-    xin[s] = *reinterpret_cast<const bf16x8_t*>(a.X + cell * a.ldx + 16 * s + 8 * h);
+  const bool active = cell0 < a.M;                    // idle waves still stage tiles
+  const long long cell = active ? cell0 + (lane & 31) : (lane & 31);
+  constexpr int NL = FMAX / 4;                        // 16-byte loads per thread per tile
+  u32x4_t r[NL];
+  tile_fetch<NL>(a.F[0][0], tid, r);
+  tile_store<NL>(wbuf, tid, r);
+  __syncthreads();
   for (int m = 0; m < a.nmembers; ++m) {
-    bf16x8_t hb[NS];
-    infer_hidden<NIN_S, NS>(a.F[m][0], a.bias[m][0], xin, hb, lane);
-    for (int l = 1; l < a.nhl; ++l) {
-      bf16x8_t hn[NS];
-      infer_hidden<NS, NS>(a.F[m][l], a.bias[m][l], hb, hn, lane);
+    bf16x8_t in[FMAX];                                // layer input slabs
 #pragma unroll
-      for (int s = 0; s < NS; ++s) hb[s] = hn[s];
-    }
-    const __bf16* F = a.F[m][a.nhl];
-    const float* bias = a.bias[m][a.nhl];
-    float* out = a.out[m] + cell * (32 * NOUT_T);
+    for (int s = 0; s < NIN_S; ++s)
+      in[s] = *reinterpret_cast<const bf16x8_t*>(a.X + cell * a.ldx + 16 * s + 8 * h);
 #pragma unroll
-    for (int ot = 0; ot < NOUT_T; ++ot) {
-      f32x16_t acc;
+    for (int l = 0; l <= NHL; ++l) {
+      const int nt = (l == NHL) ? NOUT_T : NT_H;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+      for (int t = 0; t < nt; ++t) {
+        tile_fetch<NL>(next_tile<NH, NHL, NIN_S, NOUT_T>(a, m, l, t), tid, r);
+        const f32x16_t acc = (l == 0) ? tile_mfma<NIN_S>(wbuf, in, lane) : tile_mfma<NS>(wbuf, in, lane);
+        if (l < NHL) {
+          tile_gelu_lds(acc, a.bias[m][l], t, h, lane, act);
+        } else if (active) {
+          const float* bias = a.bias[m][NHL];
+          float* out = a.out[m] + cell * (32 * NOUT_T);
 #pragma unroll
-      for (int ks = 0; ks < NS; ++ks)
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_load(F, ot * NS + ks, lane), hb[ks], acc, 0, 0, 0);
+          for (int q = 0; q < 4; ++q) {
+            const int o = t * 32 + 8 * q + 4 * h;
+            const float4 b = *reinterpret_cast<const float4*>(bias + o);
+            *reinterpret_cast<float4*>(out + o) =
+                make_float4(acc[4 * q] + b.x, acc[4 * q + 1] + b.y, acc[4 * q + 2] + b.z, acc[4 * q + 3] + b.w);
+          }
+        }
+        __syncthreads();                              // every wave is done with this tile
+        tile_store<NL>(wbuf, tid, r);
+        __syncthreads();                              // the next tile is in place
+      }
+      if (l < NHL) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int o = ot * 32 + 8 * q + 4 * h;
-        const float4 b = *reinterpret_cast<const float4*>(bias + o);
-        *reinterpret_cast<float4*>(out + o) =
-            make_float4(acc[4 * q] + b.x, acc[4 * q + 1] + b.y, acc[4 * q + 2] + b.z, acc[4 * q + 3] + b.w);
+        for (int s = 0; s < NS; ++s)                // this wave's own outputs
+          in[s] = __builtin_bit_cast(bf16x8_t, act[s * 64 + lane]);
       }
     }
   }

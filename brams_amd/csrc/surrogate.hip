@@ -103,12 +103,23 @@ int pack_frags(brams_mlp* m, hipStream_t st) {
   return 0;
 }
 
-template <int NH>
+template <int NH, int NHL>
 hipError_t launch_infer(const brams::InferArgs& a, hipStream_t st) {
   const unsigned grid = (unsigned)((a.M + 127) / 128);
-  hipLaunchKernelGGL((brams::mlp_infer<NH, BRAMS_SUR_NIN_PAD / 16, BRAMS_SUR_NOUT_PAD / 32>), dim3(grid),
+  hipLaunchKernelGGL((brams::mlp_infer<NH, NHL, BRAMS_SUR_NIN_PAD / 16, BRAMS_SUR_NOUT_PAD / 32>), dim3(grid),
                      dim3(256), 0, st, a);
   return hipGetLastError();
+}
+
+template <int NH>
+hipError_t launch_infer_nl(const brams::InferArgs& a, hipStream_t st) {
+  switch (a.nhl) {
+    case 1: return launch_infer<NH, 1>(a, st);
+    case 2: return launch_infer<NH, 2>(a, st);
+    case 3: return launch_infer<NH, 3>(a, st);
+    case 4: return launch_infer<NH, 4>(a, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 // training forward through all layers, keeping Z/A for the backward pass
@@ -212,9 +223,9 @@ extern "C" {
 
 int brams_mlp_create(int nhidden, int nlayers, int max_batch, uint64_t seed, brams_mlp** out) {
   if (!out) return brams_chem_set_error(-2, "null output");
-  if (nhidden < 64 || nhidden > 256 || nhidden % 64 || nlayers < 1 ||
-      nlayers + 1 > brams::kInferMaxLayers || max_batch < 64 || max_batch % 64)
-    return brams_chem_set_error(-2, "nhidden must be 64, 128, 192 or 256, nlayers 1..8, "
+  if (nhidden < 64 || nhidden > 256 || nhidden % 64 || nlayers < 1 || nlayers > 4 ||
+      max_batch < 64 || max_batch % 64)
+    return brams_chem_set_error(-2, "nhidden must be 64, 128, 192 or 256, nlayers 1..4, "
                                     "max_batch a multiple of 64");
   brams_mlp* m = new brams_mlp();
   m->nh = nhidden; m->nl = nlayers; m->maxb = max_batch; m->nlay = nlayers + 1;
@@ -261,7 +272,7 @@ int brams_mlp_create(int nhidden, int nlayers, int max_batch, uint64_t seed, bra
   for (int l = 0; l < m->nlay; ++l) {
     A_(dalloc(&m->W[l], (size_t)m->dout[l] * m->din[l]));
     A_(dalloc(&m->WT[l], (size_t)m->dout[l] * m->din[l]));
-    A_(dalloc(&m->Fr[l], (size_t)m->dout[l] * m->din[l]));
+    A_(dalloc(&m->Fr[l], (size_t)m->dout[l] * m->din[l] + brams::kInferPadElems));
   }
   m->Z.resize(nlayers); m->A.resize(nlayers); m->dZ.resize(nlayers); m->dZb.resize(nlayers);
   for (int l = 0; l < nlayers; ++l) {
@@ -353,11 +364,12 @@ int brams_mlp_forward_ensemble(brams_mlp* const* members, int n, const void* X, 
     }
     a.out[k] = out[k];
   }
+  if (a.nhl < 1 || a.nhl > 4) return brams_chem_set_error(-2, "fused inference takes 1..4 hidden layers");
   switch (m0->nh) {
-    case 64: SUR_TRY(launch_infer<64>(a, st)); break;
-    case 128: SUR_TRY(launch_infer<128>(a, st)); break;
-    case 192: SUR_TRY(launch_infer<192>(a, st)); break;
-    case 256: SUR_TRY(launch_infer<256>(a, st)); break;
+    case 64: SUR_TRY(launch_infer_nl<64>(a, st)); break;
+    case 128: SUR_TRY(launch_infer_nl<128>(a, st)); break;
+    case 192: SUR_TRY(launch_infer_nl<192>(a, st)); break;
+    case 256: SUR_TRY(launch_infer_nl<256>(a, st)); break;
     default: return brams_chem_set_error(-2, "unsupported hidden width");
   }
   return 0;
