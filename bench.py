@@ -122,6 +122,9 @@ def main():
     ap.add_argument("--model", default=os.path.join(ROOT, "brams_amd", "models", "relacs_tuv_surrogate.npz"))
     ap.add_argument("--spread-tol", type=float, default=None, help="override the model's spread check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="first copy this many MiB device to device (a kernel of known traffic for "
+                         "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the chemistry step has no cross-GPU exchange "
@@ -150,6 +153,20 @@ def main():
     fbuf = torch.empty((grid.ncell, NIN), dtype=torch.float32, device=dev)
     valid = torch.ones(grid.ncell, dtype=torch.bool, device=dev)
     log("workload %dx%dx%d = %d cells, %d blocks" % (args.nx, args.ny, args.nz, grid.ncell, len(wl["block_end"])))
+    if args.copy_calib_mb:
+        src = torch.ones(args.copy_calib_mb << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d MiB read + %d MiB written" % (args.copy_calib_mb, args.copy_calib_mb))
+        del src, dst
+
+    def block_spread(tag):
+        att, _, _, _ = abi.block_status(len(wl["block_end"]))
+        q = np.quantile(att, [0.5, 0.9, 0.99])
+        log("%s: attempts per block mean %.1f, p50 %.0f, p90 %.0f, p99 %.0f, max %d"
+            % (tag, att.mean(), q[0], q[1], q[2], att.max()))
 
     # ---- spin-up (untimed); step 0 is the cold step out of the seeded state
     cold = None
@@ -167,6 +184,8 @@ def main():
                     "cell_steps_per_s": grid.ncell / (time.perf_counter() - t0)}
         valid &= trusted_cells(wl["block_end"], dev) & physical_cells(ref, fbuf)
         log("spin-up step %d: %.1f ms" % (t, st.kernel_ms))
+        if t == 0:
+            block_spread("cold step")
     warm_sc = ref.sc_p.clone()
     warm_dt = ref.last_dt.clone()
 
@@ -185,6 +204,7 @@ def main():
             log("warmup %d done" % i)
         sds.sc_p.copy_(warm_sc)
         sds.last_dt.copy_(warm_dt)
+        sur.reset()
         snaps = torch.empty((K,) + tuple(sds.sc_p.shape), dtype=sds.sc_p.dtype, device=dev)
         sstats = []
         torch.cuda.synchronize()
@@ -225,6 +245,8 @@ def main():
             worst["p99"] = max(worst["p99"], p99[1][1])
             per_step.append([round(med[1][0], 5), med[0], round(p99[1][1], 5), p99[0]])
         log("fp64 step %d: %.1f ms" % (t, st.kernel_ms))
+        if t == K - 1:
+            block_spread("last fp64 step")
     fp64_value = grid.ncell * K / wall_b
 
     plain = {"kernel": "rodas3_blocks<64,false>", "bound": "fp64", "unit": "TFLOP/s",

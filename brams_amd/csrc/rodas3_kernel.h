@@ -43,6 +43,7 @@ struct StepParams {
   int* blk_failed;               // [nob]
   float* blk_forced;             // [nob] largest error of an acceptance forced at dt_min (0: none)
   const int* cell_list;          // optional: blocks are consecutive WG-sized runs of this list
+  const int* block_order;        // optional: workgroup w integrates block block_order[w]
   int list_n;                    // length of cell_list
 };
 
@@ -87,7 +88,7 @@ __device__ __forceinline__ void block_reduce(double& errmax, int& rej, int& fail
 // the surrogate step) instead of the AllocIndex partition.
 template <int WG, bool DYN, bool LIST = false>
 __global__ __launch_bounds__(WG) void rodas3_blocks(StepParams p, int block0) {
-  const int b = block0 + blockIdx.x;
+  const int b = (!LIST && p.block_order) ? p.block_order[blockIdx.x] : block0 + blockIdx.x;
   int start, ncell;
   if constexpr (LIST) {                             // fallback blocks of listed cells
     start = b * WG;

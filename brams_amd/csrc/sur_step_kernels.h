@@ -119,8 +119,8 @@ __global__ void normalize_rows(const float* raw, long long n, int ncol, const fl
 }
 
 // normalised features straight from the state (inference): X[g][128] bf16
-__global__ void sur_features(StepParams p, long long ncell, long long nrows, double floor_c,
-                             const float* mu, const float* sd, __bf16* X) {
+__global__ __launch_bounds__(256) void sur_features(StepParams p, long long ncell, long long nrows, double floor_c,
+                                                   const float* mu, const float* sd, __bf16* X) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nrows) return;
   __bf16* x = X + g * kSurNinPad;
@@ -130,13 +130,17 @@ __global__ void sur_features(StepParams p, long long ncell, long long nrows, dou
   }
   const CellState c = cell_state(p, g);
   float f[kSurNin];
+#pragma unroll
   for (int s = 0; s < MECH_NS; ++s) f[s] = (float)log10(fmax(cell_conc(p, p.sc_p, c, s), floor_c));
   f[47] = (float)c.temp;
   f[48] = (float)log(c.press);
   f[49] = (float)log10(fmax(c.vapp, 1e-12));
+#pragma unroll
   for (int n = 0; n < MECH_NPHOT; ++n) f[50 + n] = (float)p.jphoto[n + (long long)MECH_NPHOT * c.o];
   f[67] = (float)log10((double)(p.dtlt * (float)p.ndyn));
+#pragma unroll
   for (int k = 0; k < kSurNin; ++k) x[k] = (__bf16)((f[k] - mu[k]) / sd[k]);
+#pragma unroll
   for (int k = kSurNin; k < kSurNinPad; ++k) x[k] = (__bf16)0.0f;
 }
 
@@ -230,80 +234,67 @@ struct SurApplyParams {
 // Ensemble mean of the predicted log10 concentrations, acceptance checks,
 // write-back of accepted cells; rejected cells are flagged and keep their
 // state for the RODAS3 kernels.
-__global__ void sur_apply(StepParams p, SurApplyParams a, long long ncell) {
+// predicted concentration of species s (molec/cm^3) and its ensemble spread
+__device__ __forceinline__ double sur_predict(const StepParams& p, const SurApplyParams& a, const CellState& c,
+                                             long long g, int s, double& y0, float& spread) {
+  y0 = cell_conc(p, p.sc_p, c, s);
+  float lm[4];
+  float mean = 0.f;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    lm[m] = m < a.nmembers ? a.out[m][g * kSurNoutPad + s] * a.tsd[s] + a.tmu[s] : 0.f;
+    mean += lm[m];
+  }
+  mean /= (float)a.nmembers;
+  spread = 0.f;
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+    if (m < a.nmembers) spread = fmaxf(spread, fabsf(lm[m] - mean));
+  if (a.tsd[s] == 0.0f) return y0;              // species the mechanism never changes
+  // 10**(log10 max(y0, floor) + mean), as a multiplicative update
+  return fmax(y0, a.floor_c) * (double)exp10f(mean);
+}
+
+// Two passes over the species keep no per-species arrays: the checks, then
+// (accepted cells only) the prediction again and the write-back.
+__global__ __launch_bounds__(128) void sur_apply(StepParams p, SurApplyParams a, long long ncell) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ncell) return;
   const CellState c = cell_state(p, g);
   const float log_spread = log10f(1.0f + a.spread_tol);
-  double y0[MECH_NS], y1[MECH_NS];
   int bad_f = 0, bad_s = 0;
   double n0 = 0.0, n1 = 0.0, c0 = 0.0, c1 = 0.0;
   for (int s = 0; s < MECH_NS; ++s) {
-    y0[s] = cell_conc(p, p.sc_p, c, s);
-    const double l0 = log10(fmax(y0[s], a.floor_c));
-    float lm[4];
-    float mean = 0.f;
-    for (int m = 0; m < a.nmembers; ++m) {
-      lm[m] = a.out[m][g * kSurNoutPad + s] * a.tsd[s] + a.tmu[s];
-      mean += lm[m];
-    }
-    mean /= (float)a.nmembers;
-    float spread = 0.f;
-    for (int m = 0; m < a.nmembers; ++m) spread = fmaxf(spread, fabsf(lm[m] - mean));
-    double v = pow(10.0, l0 + (double)mean);
-    if (a.tsd[s] == 0.0f) v = y0[s];         // species the mechanism never changes
-    y1[s] = v;
+    double y0;
+    float spread;
+    const double v = sur_predict(p, a, c, g, s, y0, spread);
     if (!isfinite(v) || v < 0.0) bad_f = 1;
-    if (spread > log_spread && (v > a.sig_c || y0[s] > a.sig_c)) bad_s = 1;
-    n0 += a.natoms[s] * y0[s];
+    if (spread > log_spread && (v > a.sig_c || y0 > a.sig_c)) bad_s = 1;
+    n0 += a.natoms[s] * y0;
     n1 += a.natoms[s] * v;
-    c0 += a.catoms[s] * y0[s];
+    c0 += a.catoms[s] * y0;
     c1 += a.catoms[s] * v;
   }
   const int bad_n = fabs(n1 - n0) > a.n_tol * n0;
   const int bad_c = fabs(c1 - c0) > a.c_tol * c0;
-  a.flag[g] = (bad_f || bad_n || bad_c || bad_s) ? 1 : 0;
-  if (bad_f || bad_n || bad_c || bad_s) {
-    if (bad_f) atomicAdd(&a.reason[0], 1);
-    else if (bad_n) atomicAdd(&a.reason[1], 1);
-    else if (bad_c) atomicAdd(&a.reason[2], 1);
-    else atomicAdd(&a.reason[3], 1);
-    return;
-  }
-  for (int s = 0; s < MECH_NS; ++s) {
-    const long long idx = (long long)s * p.n3d + c.o;
-    if (!p.transported[s]) {
-      const float v = (float)y1[s];
-      p.sc_p[idx] = v > 0.0f ? v : 0.0f;
-    } else {
-      const float v = (float)(y1[s] * (double)p.weight[s] * c.volmol_i);
-      p.sc_p[idx] = v > 0.0f ? v : 0.0f;
-    }
-  }
-}
-
-// Initial step of each fallback block (64 consecutive list entries): the
-// smallest step carried by the partition blocks its cells belong to
-// (last_accepted_dt, rodas3:686), found by bisection of block_start.
-__global__ __launch_bounds__(64) void fallback_dt_hint(const int* __restrict__ list, int nfb,
-                                                       const int* __restrict__ block_start, int nob,
-                                                       const double* __restrict__ last_dt, double dt_max,
-                                                       double* __restrict__ fb_dt) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  double d = dt_max;
-  if (i < nfb) {
-    const int g = list[i];
-    int lo = 0, hi = nob - 1;                 // largest b with block_start[b] <= g
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (block_start[mid] <= g) lo = mid;
-      else hi = mid - 1;
-    }
-    d = fmin(d, last_dt[lo]);
-  }
+  const int why = bad_f ? 0 : bad_n ? 1 : bad_c ? 2 : bad_s ? 3 : -1;
+  a.flag[g] = why >= 0 ? 1 : 0;
+  // reason counts: one atomic per wave and reason, not one per cell
+  const unsigned long long active = __ballot(1);
+  const int leader = __ffsll((long long)active) - 1;
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) d = fmin(d, __shfl_xor(d, off, 64));
-  if (threadIdx.x == 0) fb_dt[blockIdx.x] = d;
+  for (int r = 0; r < 4; ++r) {
+    const unsigned long long m = __ballot(why == r);
+    if (m && (int)(threadIdx.x & 63) == leader) atomicAdd(&a.reason[r], __popcll(m));
+  }
+  if (why >= 0) return;
+  for (int s = 0; s < MECH_NS; ++s) {
+    double y0;
+    float spread;
+    const double v = sur_predict(p, a, c, g, s, y0, spread);
+    const float w = p.transported[s] ? (float)(v * (double)p.weight[s] * c.volmol_i) : (float)v;
+    p.sc_p[(long long)s * p.n3d + c.o] = w > 0.0f ? w : 0.0f;
+  }
 }
 
 }  // namespace brams

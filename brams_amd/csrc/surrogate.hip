@@ -395,26 +395,30 @@ int brams_mlp_gradients(brams_mlp* m, const void* X, const float* Y, int B, floa
   return 0;
 }
 
-int brams_mlp_train(brams_mlp* m, const void* X, const float* Y, int64_t N, int batch, int epochs,
-                    float lr0, uint64_t seed, float* epoch_loss) {
-  if (!m || !X || !Y || batch <= 0 || batch % 64 || batch > m->maxb || N < batch || epochs < 1)
+int brams_mlp_train_range(brams_mlp* m, const void* X, const float* Y, int64_t N, int batch, int epoch0,
+                          int nepochs, int total_epochs, float lr0, uint64_t seed, float* epoch_loss) {
+  if (!m || !X || !Y || batch <= 0 || batch % 64 || batch > m->maxb || N < batch || nepochs < 1 ||
+      epoch0 < 0 || epoch0 + nepochs > total_epochs)
     return brams_chem_set_error(-2, "bad training arguments");
   if (m->idx_n < N) {
-    if (m->idx) hipFree(m->idx);
+    if (m->idx) (void)hipFree(m->idx);
     m->idx = nullptr;
     SUR_TRY(dalloc(&m->idx, (size_t)N));
     m->idx_n = N;
   }
   const long long nb = N / batch;
-  const long long total = nb * epochs;
+  const long long total = nb * total_epochs;
   std::vector<long long> perm(N);
-  std::iota(perm.begin(), perm.end(), 0LL);
-  std::mt19937_64 rng(seed);
-  long long t = 0;
-  for (int ep = 0; ep < epochs; ++ep) {
+  for (int e = 0; e < nepochs; ++e) {
+    const int ep = epoch0 + e;
+    // the permutation of an epoch depends on (seed, epoch) only, so a
+    // schedule run in several calls equals one run in a single call
+    std::iota(perm.begin(), perm.end(), 0LL);
+    std::mt19937_64 rng(seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(ep + 1));
     std::shuffle(perm.begin(), perm.end(), rng);
     SUR_TRY(hipMemcpy(m->idx, perm.data(), sizeof(long long) * N, hipMemcpyHostToDevice));
     SUR_TRY(fill(m->loss, 1, 0.f, nullptr));
+    long long t = (long long)ep * nb;
     for (long long b = 0; b < nb; ++b, ++t) {
       hipLaunchKernelGGL(brams::gather_rows, dim3(batch), dim3(128), 0, nullptr, (const __bf16*)X,
                          BRAMS_SUR_NIN_PAD, Y, BRAMS_SUR_NOUT_PAD, m->idx + b * batch, batch, m->Xb,
@@ -428,10 +432,15 @@ int brams_mlp_train(brams_mlp* m, const void* X, const float* Y, int64_t N, int 
     }
     float l = 0.f;
     SUR_TRY(hipMemcpy(&l, m->loss, sizeof(float), hipMemcpyDeviceToHost));
-    if (epoch_loss) epoch_loss[ep] = l / (float)nb;
+    if (epoch_loss) epoch_loss[e] = l / (float)nb;
   }
   SUR_TRY(hipDeviceSynchronize());
   return 0;
+}
+
+int brams_mlp_train(brams_mlp* m, const void* X, const float* Y, int64_t N, int batch, int epochs,
+                    float lr0, uint64_t seed, float* epoch_loss) {
+  return brams_mlp_train_range(m, X, Y, N, batch, 0, epochs, epochs, lr0, seed, epoch_loss);
 }
 
 }  // extern "C"

@@ -28,6 +28,8 @@ def _bind():
                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.brams_mlp_train.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int,
                                   C.c_float, C.c_uint64, C.POINTER(C.c_float)]
+    L.brams_mlp_train_range.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, C.c_float, C.c_uint64, C.POINTER(C.c_float)]
     L._mlp_bound = True
     return L
 
@@ -82,9 +84,18 @@ class MLP:
                                               loss.ctypes.data_as(C.POINTER(C.c_float))), "gradients")
         return float(loss[0]), g
 
-    def train(self, X, Y, batch=8192, epochs=1, lr0=1e-3, seed=0):
+    def train(self, X, Y, batch=8192, epochs=1, lr0=1e-3, seed=0, chunk=None, log=None):
+        """Adam with a cosine schedule over `epochs`; with `chunk`, the run is
+        split into calls of that many epochs and `log` reports each."""
         losses = np.zeros(epochs, np.float32)
-        abi.check(_bind().brams_mlp_train(self.h, C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()),
-                                          X.shape[0], batch, epochs, lr0, seed,
-                                          losses.ctypes.data_as(C.POINTER(C.c_float))), "train")
+        chunk = chunk or epochs
+        for e0 in range(0, epochs, chunk):
+            n = min(chunk, epochs - e0)
+            out = np.zeros(n, np.float32)
+            abi.check(_bind().brams_mlp_train_range(
+                self.h, C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()), X.shape[0], batch, e0, n,
+                epochs, lr0, seed, out.ctypes.data_as(C.POINTER(C.c_float))), "train")
+            losses[e0:e0 + n] = out
+            if log:
+                log("  epochs %d-%d of %d: loss %.5g" % (e0 + 1, e0 + n, epochs, out[-1]))
         return losses

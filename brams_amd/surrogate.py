@@ -86,6 +86,8 @@ def _bind():
     L.brams_sur_normalize.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_int, C.c_int]
     L.brams_sur_step.argtypes = [dev, C.POINTER(SurModel), C.POINTER(SurStats)]
+    L.brams_sur_reset.argtypes = []
+    L.brams_sur_reset.restype = None
     L._sur_bound = True
     return L
 
@@ -250,7 +252,7 @@ def train_ensemble(feat, targ, nmembers=3, nhidden=256, nlayers=4, epochs=50, ba
         t0 = time.time()
         net = MLP(nhidden, nlayers, max_batch=batch, seed=seed * 100 + k + 1)
         losses = net.train(X[:n] if n == X.shape[0] else X, Y, batch=batch, epochs=epochs, lr0=lr0,
-                           seed=seed * 100 + k + 11)
+                           seed=seed * 100 + k + 11, chunk=10, log=log)
         model.params[k] = net.get_params()
         curves.append(losses.tolist())
         log("train: member %d, %d epochs x %d batches, loss %.4g -> %.4g, %.1f s"
@@ -293,6 +295,11 @@ class SurrogateStep:
         m.sig_c = model.sig_c
         m.spread_tol, m.n_tol, m.c_tol = model.spread_tol, model.n_tol, model.c_tol
         self.cmodel = m
+        self.reset()
+
+    def reset(self):
+        """Forget the fallback's per-cell memory (after restoring a state)."""
+        _bind().brams_sur_reset()
 
     def step(self):
         s = SurStats()

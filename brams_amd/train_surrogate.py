@@ -121,7 +121,7 @@ def main():
     ap.add_argument("--ny", type=int, default=200)
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--members", type=int, default=3)
-    ap.add_argument("--epochs", type=int, default=50)
+    ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=os.path.join(ROOT, "brams_amd", "models", "relacs_tuv_surrogate.npz"))

@@ -66,6 +66,12 @@ int brams_mlp_gradients(brams_mlp *m, const void *X, const float *Y, int B, floa
  * NULL) receives the mean training loss of each epoch. */
 int brams_mlp_train(brams_mlp *m, const void *X, const float *Y, int64_t N, int batch, int epochs,
                     float lr0, uint64_t seed, float *epoch_loss);
+/* Epochs [epoch0, epoch0 + nepochs) of a total_epochs run of brams_mlp_train
+ * (same cosine schedule, same per-epoch permutations): a long run split
+ * into calls, e.g. to report progress.  epoch_loss gets nepochs values. */
+int brams_mlp_train_range(brams_mlp *m, const void *X, const float *Y, int64_t N, int batch,
+                          int epoch0, int nepochs, int total_epochs, float lr0, uint64_t seed,
+                          float *epoch_loss);
 
 /* ------------------------------------------------------------------ step
  * The surrogate chemistry step on one grid, device-resident arrays as in
@@ -108,6 +114,10 @@ int brams_sur_normalize(const float *raw, int64_t n, int ncol, const float *mu, 
 /* Surrogate + fallback step: features, ensemble inference, checks, write-back
  * of accepted cells, FP64 RODAS3 on the rejected ones. */
 int brams_sur_step(const brams_chem_dev_args *a, const brams_sur_model *m, brams_sur_stats *st);
+/* Forget the fallback's per-cell memory of the last accepted step; the next
+ * brams_sur_step starts it again from the partition blocks'
+ * last_accepted_dt.  For a new run or a restored state. */
+void brams_sur_reset(void);
 
 #ifdef __cplusplus
 }

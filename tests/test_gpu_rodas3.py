@@ -50,3 +50,25 @@ def test_host_entry_matches_device_entry():
     for a, b in zip(sc_p, gpu):
         assert np.array_equal(a, b)
     assert np.array_equal(last_dt, dt_gpu)
+
+
+def test_longest_first_block_order_changes_no_result():
+    """The second call on a partition launches its blocks ordered by the
+    first call's attempts; every block's result must stay bit-identical."""
+    from brams_amd import abi
+    from brams_amd.chem import DeviceStep
+    print("case block order, 24x24x10 seed 5", flush=True)
+    abi.lib().brams_chem_finalize()                     # no attempts history: partition order
+    grid = Grid(24, 24, 10)
+    wl = make_workload(grid, seed=5, sza="split", plume_sigma=4.0)
+    atol, rtol = tolerances("baseline")
+    ds = DeviceStep(wl, atol, rtol)
+    ds.step()
+    first = ds.sc_p.cpu().numpy().copy()
+    first_dt = ds.last_dt.cpu().numpy().copy()
+    att, _, _, _ = abi.block_status(len(wl["block_end"]))
+    assert att.max() > att.min()                        # the order is not the identity
+    ds.reset()
+    ds.step()                                           # ordered by the first call's attempts
+    assert np.array_equal(first, ds.sc_p.cpu().numpy())
+    assert np.array_equal(first_dt, ds.last_dt.cpu().numpy())
