@@ -27,8 +27,8 @@ sub-steps and attempts of every cell (RODAS3), 1.356 MFLOP per cell for the
 stream.
 
 cpu_baseline: the oracle (C restatement of the reference step) on a bounded
-sample of the warm state (every 5th column in i and j, all 40 levels: 64 000
-cells, one step, step size restarted at dt), all host cores.
+sample of the warm state (every 3rd column in i and j, all 40 levels:
+179 560 cells, one step, step size restarted at dt), all host cores.
 """
 import argparse
 import json
@@ -61,7 +61,7 @@ def flop_model():
     return f_call, f_sub, f_att
 
 
-def sample_workload(wl, stride=5):
+def sample_workload(wl, stride=3):
     """Every stride-th column of a workload, as a smaller grid of its own."""
     import numpy as np
     from brams_amd.workload import Grid, alloc_index
@@ -86,19 +86,27 @@ def sample_workload(wl, stride=5):
 
 MLP_FLOP_PER_CELL = 3 * 2 * (68 * 256 + 3 * 256 * 256 + 256 * 47)
 BF16_PEAK_TFLOPS = 2500.0
+# HBM traffic measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by
+# the counters' reading of a 1 GiB device copy: x2.0 and x1.0) on this
+# workload, profiles/rodas3_hbm_traffic_pmc.md: the RODAS3 kernels move
+# 29.4 KB read + 8.7 KB written per cell-attempt in every launch measured
+# (the spill traffic of the per-lane LU), the fused MLP 2.94 GB per launch
+# over 1.6M cells (1840 B per cell).
+RODAS3_BYTES_PER_CELL_ATTEMPT = 29400 + 8700
+MLP_BYTES_PER_CELL = 1840
 
 
 def cpu_baseline(wl, atol, rtol):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    sub = sample_workload(wl)
+    sub = sample_workload(wl, 3)
     cores = len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
     hs, st, mask, rc = oracle_lib.run_oracle(sub, atol, rtol, nthreads=cores)
     dt = time.perf_counter() - t0
     return {"value": sub["grid"].ncell / dt, "unit": "cell-steps/s", "cores": cores,
             "kind": "port",
-            "sample": "oracle (C restatement, gcc -O2, pthreads over blocks) on every 5th column of "
+            "sample": "oracle (C restatement, gcc -O2, pthreads over blocks) on every 3rd column of "
                       "the warm state in i and j: %d cells x 1 step, %.1f s wall, %d attempts"
                       % (sub["grid"].ncell, dt, st.attempts)}
 
@@ -183,7 +191,7 @@ def main():
                     "substeps_per_cell": st.cell_substeps / st.cells,
                     "cell_steps_per_s": grid.ncell / (time.perf_counter() - t0)}
         valid &= trusted_cells(wl["block_end"], dev) & physical_cells(ref, fbuf)
-        log("spin-up step %d: %.1f ms" % (t, st.kernel_ms))
+        log("spin-up step %d: %.1f ms (%d cell-attempts)" % (t, st.kernel_ms, st.cell_attempts))
         if t == 0:
             block_spread("cold step")
     warm_sc = ref.sc_p.clone()
@@ -214,15 +222,15 @@ def main():
             s = sur.step()
             snaps[t].copy_(sds.sc_p)
             sstats.append(s)
-            log("surrogate step %d: %.1f ms (mlp %.2f, fallback %.1f ms, %d fallback cells)"
-                % (t, s.total_ms, s.mlp_ms, s.fallback_ms, s.fallback_cells))
+            log("surrogate step %d: %.1f ms (mlp %.2f, fallback %.1f ms, %d fallback cells, %d cell-attempts)"
+                % (t, s.total_ms, s.mlp_ms, s.fallback_ms, s.fallback_cells, s.fallback_cell_attempts))
         torch.cuda.synchronize()
         wall_a = time.perf_counter() - t0
 
     # ---- leg B: FP64 rollout from the same warm state
     ref.sc_p.copy_(warm_sc)
     ref.last_dt.copy_(warm_dt)
-    wall_b, b_main_ms, b_flops_main, b_kernel_ms = 0.0, 0.0, 0.0, 0.0
+    wall_b, b_main_ms, b_flops_main, b_kernel_ms, b_att_main = 0.0, 0.0, 0.0, 0.0, 0.0
     worst = {"median": 0.0, "p99": 0.0}
     per_step = []
     for t in range(K):
@@ -236,6 +244,7 @@ def main():
         b_main_ms += st.main_ms
         b_kernel_ms += st.kernel_ms
         b_flops_main += rodas3_flops(st.cells, st.cell_substeps, st.cell_attempts) * st.main_cells / st.cells
+        b_att_main += st.cell_attempts * st.main_cells / st.cells
         valid &= trusted_cells(wl["block_end"], dev) & physical_cells(ref, fbuf)
         if sur is not None:
             err = species_errors(snaps[t], ref.sc_p, grid, valid=valid)
@@ -244,14 +253,16 @@ def main():
             worst["median"] = max(worst["median"], med[1][0])
             worst["p99"] = max(worst["p99"], p99[1][1])
             per_step.append([round(med[1][0], 5), med[0], round(p99[1][1], 5), p99[0]])
-        log("fp64 step %d: %.1f ms" % (t, st.kernel_ms))
+        log("fp64 step %d: %.1f ms (main kernel %.1f ms, %d cell-attempts)"
+            % (t, st.kernel_ms, st.main_ms, st.cell_attempts))
         if t == K - 1:
             block_spread("last fp64 step")
     fp64_value = grid.ncell * K / wall_b
 
     plain = {"kernel": "rodas3_blocks<64,false>", "bound": "fp64", "unit": "TFLOP/s",
              "avg_ms": b_main_ms / K, "achieved": b_flops_main / (b_main_ms * 1e-3) / 1e12,
-             "peak": FP64_PEAK_TFLOPS}
+             "peak": FP64_PEAK_TFLOPS, "traffic": RODAS3_BYTES_PER_CELL_ATTEMPT * b_att_main / K,
+             "algorithmic_bytes": 528 * grid.ncell}
     plain["frac"] = plain["achieved"] / plain["peak"]
     if sur is None:
         value, wall, path = fp64_value, wall_b, "plain FP64 RODAS3 (GPU), warm rollout"
@@ -263,12 +274,15 @@ def main():
         mlp_ms = sum(s.mlp_ms for s in sstats)
         fb_flops = sum(rodas3_flops(s.fallback_cells, s.fallback_cell_substeps, s.fallback_cell_attempts)
                        for s in sstats)
+        fb_att = sum(s.fallback_cell_attempts for s in sstats)
         fallback = {"kernel": "rodas3_blocks<64,false,true>", "bound": "fp64", "unit": "TFLOP/s",
                     "avg_ms": fb_ms / K, "achieved": fb_flops / max(fb_ms * 1e-3, 1e-9) / 1e12,
-                    "peak": FP64_PEAK_TFLOPS}
+                    "peak": FP64_PEAK_TFLOPS, "traffic": RODAS3_BYTES_PER_CELL_ATTEMPT * fb_att / K,
+                    "algorithmic_bytes": 528 * sum(s.fallback_cells for s in sstats) / K}
         fallback["frac"] = fallback["achieved"] / fallback["peak"]
         mlp = {"kernel": "mlp_infer<256,4,8,2>", "bound": "mfma", "unit": "TFLOP/s",
-               "avg_ms": mlp_ms / K,
+               "avg_ms": mlp_ms / K, "traffic": MLP_BYTES_PER_CELL * grid.ncell,
+               "algorithmic_bytes": (256 + 3 * 256) * grid.ncell,
                "achieved": MLP_FLOP_PER_CELL * grid.ncell * K / (mlp_ms * 1e-3) / 1e12,
                "peak": BF16_PEAK_TFLOPS}
         mlp["frac"] = mlp["achieved"] / mlp["peak"]
@@ -296,7 +310,10 @@ def main():
             "spread_tol": model.spread_tol, "n_tol": model.n_tol, "c_tol": model.c_tol,
         }
     roofline = {"bound": "mfma", "achieved": dominant["achieved"], "peak": dominant["peak"],
-                "unit": dominant["unit"], "frac": dominant["frac"], "traffic": None,
+                "unit": dominant["unit"], "frac": dominant["frac"], "traffic": dominant["traffic"],
+                "algorithmic_bytes": dominant["algorithmic_bytes"],
+                "traffic_note": "HBM bytes per launch: PMC-measured bytes per cell-attempt (RODAS3) or per "
+                                "cell (MLP) x this run's counts, profiles/rodas3_hbm_traffic_pmc.md",
                 "kernel": dominant["kernel"], "avg_ms": dominant["avg_ms"],
                 "peak_note": "FP64 vector for the RODAS3 kernels (no MFMA issued; MI355X FP64 vector "
                              "and matrix peaks are both 78.6 TFLOP/s), dense BF16 MFMA for mlp_infer",

@@ -441,6 +441,7 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
   HIP_TRY(hipEventRecord(ev[2], st));
   // 3. checks, write-back of accepted cells, ordered list of rejected ones
   ap.nmembers = m->nmembers;
+  ap.nrows = nrows;
   ap.tmu = m->targ_mu; ap.tsd = m->targ_sd;
   ap.natoms = m->natoms; ap.catoms = m->catoms;
   ap.floor_c = m->floor_c; ap.sig_c = m->sig_c;

@@ -271,7 +271,8 @@ struct InferArgs {
   int nmembers, nhl;          // members, hidden layers
   const __bf16* F[kInferMaxMembers][kInferMaxLayers];  // packed weights
   const float* bias[kInferMaxMembers][kInferMaxLayers];
-  float* out[kInferMaxMembers];                        // [M][NOUT] f32
+  float* out[kInferMaxMembers];                        // f32 outputs of each member
+  long long ldo;              // 0: row-major [M][NOUT]; else species-major, out[o * ldo + cell]
 };
 
 // Weight tiles are staged through LDS: the 256 threads of a workgroup load
@@ -394,13 +395,21 @@ __global__ __launch_bounds__(256) void mlp_infer(InferArgs a) {
           tile_gelu_lds(acc, a.bias[m][l], t, h, lane, act);
         } else if (active) {
           const float* bias = a.bias[m][NHL];
-          float* out = a.out[m] + cell * (32 * NOUT_T);
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int o = t * 32 + 8 * q + 4 * h;
             const float4 b = *reinterpret_cast<const float4*>(bias + o);
-            *reinterpret_cast<float4*>(out + o) =
-                make_float4(acc[4 * q] + b.x, acc[4 * q + 1] + b.y, acc[4 * q + 2] + b.z, acc[4 * q + 3] + b.w);
+            const float4 v = make_float4(acc[4 * q] + b.x, acc[4 * q + 1] + b.y, acc[4 * q + 2] + b.z,
+                                         acc[4 * q + 3] + b.w);
+            if (a.ldo) {                              // 32 consecutive cells per store: coalesced
+              float* out = a.out[m] + cell;
+              out[(o + 0) * a.ldo] = v.x;
+              out[(o + 1) * a.ldo] = v.y;
+              out[(o + 2) * a.ldo] = v.z;
+              out[(o + 3) * a.ldo] = v.w;
+            } else {
+              *reinterpret_cast<float4*>(a.out[m] + cell * (32 * NOUT_T) + o) = v;
+            }
           }
         }
         __syncthreads();                              // every wave is done with this tile

@@ -217,7 +217,8 @@ __global__ __launch_bounds__(256) void compact_write(const int* flag, long long 
 }
 
 struct SurApplyParams {
-  const float* out[4];        // member outputs [nrows][64] (normalised increments)
+  const float* out[4];        // member outputs [64][nrows] (normalised increments)
+  long long nrows;
   int nmembers;
   const float* tmu;           // [47] target mean / std
   const float* tsd;
@@ -242,7 +243,7 @@ __device__ __forceinline__ double sur_predict(const StepParams& p, const SurAppl
   float mean = 0.f;
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
-    lm[m] = m < a.nmembers ? a.out[m][g * kSurNoutPad + s] * a.tsd[s] + a.tmu[s] : 0.f;
+    lm[m] = m < a.nmembers ? a.out[m][s * a.nrows + g] * a.tsd[s] + a.tmu[s] : 0.f;
     mean += lm[m];
   }
   mean /= (float)a.nmembers;

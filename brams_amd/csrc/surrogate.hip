@@ -338,8 +338,12 @@ int brams_mlp_set_params(brams_mlp* m, const float* host, int64_t n) {
   return 0;
 }
 
-int brams_mlp_forward_ensemble(brams_mlp* const* members, int n, const void* X, int64_t M,
-                               float* const* out, void* stream) {
+}  // extern "C"
+
+namespace {
+
+int infer(brams_mlp* const* members, int n, const void* X, int64_t M, float* const* out, bool species_major,
+          void* stream) {
   if (!members || n < 1 || n > brams::kInferMaxMembers || !X || !out || M <= 0 || M % 64)
     return brams_chem_set_error(-2, "bad forward arguments");
   hipStream_t st = (hipStream_t)stream;
@@ -350,6 +354,7 @@ int brams_mlp_forward_ensemble(brams_mlp* const* members, int n, const void* X, 
   a.ldx = BRAMS_SUR_NIN_PAD;
   a.nmembers = n;
   a.nhl = m0 ? m0->nl : 0;
+  a.ldo = species_major ? M : 0;
   for (int k = 0; k < n; ++k) {
     brams_mlp* m = members[k];
     if (!m || !out[k] || m->nh != m0->nh || m->nl != m0->nl)
@@ -375,8 +380,17 @@ int brams_mlp_forward_ensemble(brams_mlp* const* members, int n, const void* X, 
   return 0;
 }
 
+}  // namespace
+
+extern "C" {
+
+int brams_mlp_forward_ensemble(brams_mlp* const* members, int n, const void* X, int64_t M,
+                               float* const* out, void* stream) {
+  return infer(members, n, X, M, out, true, stream);
+}
+
 int brams_mlp_forward(brams_mlp* m, const void* X, int64_t M, float* out, void* stream) {
-  int rc = brams_mlp_forward_ensemble(&m, 1, X, M, &out, stream);
+  int rc = infer(&m, 1, X, M, &out, false, stream);
   if (rc) return rc;
   SUR_TRY(hipStreamSynchronize((hipStream_t)stream));
   return 0;

@@ -48,7 +48,8 @@ int brams_mlp_set_params(brams_mlp *m, const float *host, int64_t n);
 int brams_mlp_forward(brams_mlp *m, const void *X, int64_t M, float *out, void *stream);
 
 /* Forward pass of n <= 4 networks of one shape over the same rows in one
- * launch: out[k] [M][BRAMS_SUR_NOUT_PAD] f32 for member k.  Stream-ordered
+ * launch: out[k] [BRAMS_SUR_NOUT_PAD][M] f32 (species-major, so that a
+ * per-cell consumer reads it coalesced) for member k.  Stream-ordered
  * (returns before the kernel ends). */
 int brams_mlp_forward_ensemble(brams_mlp *const *members, int n, const void *X, int64_t M,
                                float *const *out, void *stream);

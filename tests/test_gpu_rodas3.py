@@ -72,3 +72,25 @@ def test_longest_first_block_order_changes_no_result():
     ds.step()                                           # ordered by the first call's attempts
     assert np.array_equal(first, ds.sc_p.cpu().numpy())
     assert np.array_equal(first_dt, ds.last_dt.cpu().numpy())
+
+
+@pytest.mark.parametrize("nx,nz,dt", [(98, 4, 60.0), (50, 6, 30.0)])
+def test_nested_grid_shapes_match_oracle(nx, nz, dt):
+    """The nested grids of the three-grid workload (98x98 and 50x50 columns,
+    dt 60 s and 30 s) at reduced depth: AllocIndex gives them 65-cell blocks
+    (16 and 24 of them), which run as 128-lane workgroups with a two-wave
+    step-size reduction; split SZA and a plume, atol 1e-12 ppb."""
+    print("case nested nx=%d nz=%d dt=%g" % (nx, nz, dt), flush=True)
+    grid = Grid(nx, nx, nz, dtlt=dt)
+    wl = make_workload(grid, seed=21, sza="split", plume_sigma=8.0)
+    assert max(wl["block_end"]) == 65
+    atol, rtol = tolerances("baseline")
+    gpu, dt_gpu, rc, st = _device_run(wl, atol, rtol)
+    hs, ost, mask, orc = oracle_lib.run_oracle(wl, atol, rtol)
+    rep = parity_report(gpu, hs.sc_p, grid, wl)
+    print("gpu attempts %d | oracle %d | worst %.3g nfail %d" % (st.attempts, ost.attempts, rep["worst"],
+                                                                 rep["nfail"]))
+    assert rc == 0 and orc == 0
+    assert st.attempts == ost.attempts and st.substeps == ost.substeps
+    assert rep["nfail"] == 0, rep
+    np.testing.assert_allclose(dt_gpu, hs.last_dt, rtol=1e-6)
