@@ -19,9 +19,9 @@ def _device_run(wl, atol, rtol, **kw):
 
 @pytest.mark.parametrize("tol", ["baseline", "reference"])
 @pytest.mark.parametrize("seed", [0, 1])
-def test_cfg1_device_matches_oracle(seed, tol):
+def test_16x16x10_device_matches_oracle(seed, tol):
     """16x16x10 block (2560 cells, 40 blocks), random SZA 0-120 deg."""
-    print("case cfg1 seed=%d tol=%s" % (seed, tol), flush=True)
+    print("case 16x16x10 seed=%d tol=%s" % (seed, tol), flush=True)
     grid = Grid(16, 16, 10)
     wl = make_workload(grid, seed=seed)
     atol, rtol = tolerances(tol)

@@ -170,7 +170,7 @@ def _step(seed, tol, **kw):
 @pytest.mark.ref
 @pytest.mark.parametrize("tol", ["reference", "baseline"])
 def test_step_with_reference_sparse(tol):
-    """cfg1 with the oracle's fixed-order LU == with the reference's Sparse1.3."""
+    """16x16x10 workload: the oracle's fixed-order LU == the reference's Sparse1.3."""
     wl, (hs, st, mask, rc) = _step(0, tol)
     _, (hr, sr, maskr, rcr) = _step(0, tol, reference_sparse=True)
     assert rc == 0 and rcr == 0
