@@ -6,7 +6,10 @@ gradient of its training loss, used to check the GPU kernels
 same points as the GPU (weights, inputs, hidden activations and the
 back-propagated deltas that feed the matrix cores) and accumulates in FP64,
 so the remaining GPU/oracle difference is the GPU's FP32 accumulation.
-The surrogate has no counterpart in the reference.
+With exact=True the rounding is off and the functions are the plain FP64
+network and its exact loss gradient, which tests/test_mlp_oracle.py pins
+with central finite differences.  The surrogate has no counterpart in the
+reference.
 """
 import numpy as np
 from scipy.special import erf
@@ -56,25 +59,37 @@ def pack(layers):
     return np.concatenate([np.concatenate([W.ravel(), b]) for W, b in layers])
 
 
-def forward(layers, X):
+def _identity(x):
+    return np.array(x, dtype=np.float64)
+
+
+def forward(layers, X, exact=False):
     """Returns the output and the per-layer (input activation, pre-activation)."""
+    rnd = _identity if exact else bf16
     nl = len(layers)
-    a = bf16(X)
+    a = rnd(X)
     cache = []
     for l, (W, b) in enumerate(layers):
-        Wb = bf16(W)
+        Wb = rnd(W)
         if l == nl - 1:
             Wb[NOUT:] = 0.0
         z = a @ Wb.T + b
         cache.append((a, z))
         if l < nl - 1:
-            a = bf16(gelu(z))
+            a = rnd(gelu(z))
     return z, cache
 
 
-def loss_and_grads(layers, X, Y):
+def loss(layers, X, Y, exact=False):
+    out, _ = forward(layers, X, exact)
+    e = out[:, :NOUT] - Y[:, :NOUT]
+    return np.sum(e * e) / (X.shape[0] * NOUT)
+
+
+def loss_and_grads(layers, X, Y, exact=False):
     """Mean squared error over the first NOUT outputs and its gradient."""
-    out, cache = forward(layers, X)
+    rnd = _identity if exact else bf16
+    out, cache = forward(layers, X, exact)
     B = X.shape[0]
     e = out[:, :NOUT] - Y[:, :NOUT]
     loss = np.sum(e * e) / (B * NOUT)
@@ -83,13 +98,13 @@ def loss_and_grads(layers, X, Y):
     grads = [None] * len(layers)
     for l in range(len(layers) - 1, -1, -1):
         a, z = cache[l]
-        db_ = bf16(d)
+        db_ = rnd(d)
         dW = db_.T @ a
         dbias = d.sum(axis=0)
         grads[l] = (dW, dbias)
         if l == 0:
             break
-        W = bf16(layers[l][0])
+        W = rnd(layers[l][0])
         if l == len(layers) - 1:
             W[NOUT:] = 0.0
         da = db_ @ W
