@@ -88,12 +88,14 @@ MLP_FLOP_PER_CELL = 3 * 2 * (68 * 256 + 3 * 256 * 256 + 256 * 47)
 BF16_PEAK_TFLOPS = 2500.0
 # HBM traffic measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by
 # the counters' reading of a 1 GiB device copy: x2.0 and x1.0) on this
-# workload, profiles/rodas3_hbm_traffic_pmc.md: the RODAS3 kernels move
-# 29.4 KB read + 8.7 KB written per cell-attempt in every launch measured
-# (the spill traffic of the per-lane LU), the fused MLP 2.94 GB per launch
-# over 1.6M cells (1840 B per cell).
-RODAS3_BYTES_PER_CELL_ATTEMPT = 29400 + 8700
-MLP_BYTES_PER_CELL = 1840
+# workload, profiles/rodas3_hbm_traffic_pmc.md.  Per cell-attempt, the same
+# in every launch measured (the spill traffic of the per-lane LU): the
+# partition kernel (four waves per SIMD) 48.9 KB read + 14.4 KB written, the
+# fallback list kernel (one wave per SIMD) 29.3 KB + 8.7 KB; the fused MLP
+# 2.90 GB per launch over 1.6M cells (1815 B per cell).
+RODAS3_BYTES_PER_CELL_ATTEMPT = 48900 + 14400
+RODAS3_LIST_BYTES_PER_CELL_ATTEMPT = 29300 + 8700
+MLP_BYTES_PER_CELL = 1815
 
 
 def cpu_baseline(wl, atol, rtol):
@@ -259,7 +261,7 @@ def main():
             block_spread("last fp64 step")
     fp64_value = grid.ncell * K / wall_b
 
-    plain = {"kernel": "rodas3_blocks<64,false>", "bound": "fp64", "unit": "TFLOP/s",
+    plain = {"kernel": "rodas3_blocks<64,false,false>", "bound": "fp64", "unit": "TFLOP/s",
              "avg_ms": b_main_ms / K, "achieved": b_flops_main / (b_main_ms * 1e-3) / 1e12,
              "peak": FP64_PEAK_TFLOPS, "traffic": RODAS3_BYTES_PER_CELL_ATTEMPT * b_att_main / K,
              "algorithmic_bytes": 528 * grid.ncell}
@@ -277,7 +279,7 @@ def main():
         fb_att = sum(s.fallback_cell_attempts for s in sstats)
         fallback = {"kernel": "rodas3_blocks<64,false,true>", "bound": "fp64", "unit": "TFLOP/s",
                     "avg_ms": fb_ms / K, "achieved": fb_flops / max(fb_ms * 1e-3, 1e-9) / 1e12,
-                    "peak": FP64_PEAK_TFLOPS, "traffic": RODAS3_BYTES_PER_CELL_ATTEMPT * fb_att / K,
+                    "peak": FP64_PEAK_TFLOPS, "traffic": RODAS3_LIST_BYTES_PER_CELL_ATTEMPT * fb_att / K,
                     "algorithmic_bytes": 528 * sum(s.fallback_cells for s in sstats) / K}
         fallback["frac"] = fallback["achieved"] / fallback["peak"]
         mlp = {"kernel": "mlp_infer<256,4,8,2>", "bound": "mfma", "unit": "TFLOP/s",

@@ -147,6 +147,95 @@ def make_dataset(grid, seeds, steps, fraction, atol, rtol, sza_step=0.5, log=pri
     return torch.cat(feats), torch.cat(targs)
 
 
+def make_rollout_dataset(model, grid, seeds, steps, fraction, atol, rtol, spinup=6, sza_step=0.5,
+                         log=print, rng_seed=1):
+    """Cell-steps on the surrogate's own trajectories (dataset aggregation).
+
+    For every seed the seeded state is advanced by `spinup` FP64 steps and
+    then by `steps` surrogate+fallback steps; before each of those, the FP64
+    RODAS3 step of the state the surrogate has reached gives the target, so
+    the network learns from the states its own errors lead to.  Returns raw
+    device tensors like make_dataset."""
+    import torch
+    from .chem import DeviceStep
+    from .workload import jphoto_from_sza, make_workload
+    L = _bind()
+    feats, targs = [], []
+    ncell = grid.ncell
+    gen = torch.Generator(device="cpu").manual_seed(rng_seed)
+    for seed in seeds:
+        t0 = time.time()
+        wl = make_workload(grid, seed=seed, sza="split", plume_sigma=20.0, plume_factor=50.0)
+        sds = DeviceStep(wl, atol, rtol)
+        ref = DeviceStep(wl, atol, rtol)
+        sur = SurrogateStep(model, sds)
+        fbuf = torch.empty((ncell, NIN), dtype=torch.float32, device=sds.dev)
+        tbuf = torch.empty((ncell, NOUT), dtype=torch.float32, device=sds.dev)
+        valid = torch.ones(ncell, dtype=torch.bool, device=sds.dev)
+        fb = 0.0
+        for t in range(spinup + steps):
+            jph = torch.from_numpy(np.ascontiguousarray(
+                jphoto_from_sza(grid, wl["sza_col"] + sza_step * t).ravel(order="F"))).to(sds.dev)
+            sds.args.jphoto = jph.data_ptr()
+            if t < spinup:
+                sds.step()
+                valid &= trusted_cells(wl["block_end"], sds.dev)
+                continue
+            ref.args.jphoto = jph.data_ptr()
+            ref.sc_p.copy_(sds.sc_p)
+            ref.last_dt.copy_(sds.last_dt)
+            before = ref.sc_p.clone()
+            abi.check(L.brams_sur_features_raw(C.byref(ref.args), fbuf.data_ptr(), FLOOR_C), "features")
+            ref.step()
+            abi.check(L.brams_sur_targets_raw(C.byref(ref.args), before.data_ptr(), tbuf.data_ptr(),
+                                              FLOOR_C), "targets")
+            ok = valid & trusted_cells(wl["block_end"], sds.dev)
+            ok &= torch.isfinite(fbuf).all(1) & torch.isfinite(tbuf).all(1)
+            ok &= ((fbuf[:, :NOUT] < LOG10_C_MAX) & (fbuf[:, :NOUT] + tbuf < LOG10_C_MAX)).all(1)
+            keep = torch.randperm(ncell, generator=gen)[:int(fraction * ncell)].to(sds.dev)
+            keep = keep[ok[keep]]
+            feats.append(fbuf.index_select(0, keep))
+            targs.append(tbuf.index_select(0, keep))
+            s = sur.step()
+            fb += s.fallback_cells / s.cells
+            del jph, before
+        log("rollout dataset: seed %d, %d surrogate steps, %.1f s, mean fallback %.3f"
+            % (seed, steps, time.time() - t0, fb / max(steps, 1)))
+        del sds, ref, sur
+    return torch.cat(feats), torch.cat(targs)
+
+
+def finetune_ensemble(model, feat, targ, epochs=50, batch=8192, lr0=3e-4, log=print, seed=7, zmax=30.0):
+    """Continue training every member on a raw dataset, keeping the model's
+    normalisation (the inference features and targets are unchanged).
+    Samples with a z-score beyond zmax are left out: on the surrogate's own
+    trajectories a species driven to the floor can regrow by many decades in
+    the FP64 step, and a few such squared errors would dominate the loss."""
+    import torch
+    X = normalise(feat, model.feat_mu, model.feat_sd, NIN_PAD, True)
+    Y = normalise(targ, model.targ_mu, model.targ_sd, NOUT_PAD, False)
+    ok = (Y.abs() <= zmax).all(1) & (X.float().abs() <= zmax).all(1)
+    n0 = X.shape[0]
+    idx = torch.nonzero(ok).squeeze(1)
+    X, Y = X.index_select(0, idx), Y.index_select(0, idx)
+    log("finetune: %d of %d samples within |z| <= %g" % (X.shape[0], n0, zmax))
+    curves = []
+    for k in range(model.nmembers):
+        t0 = time.time()
+        net = MLP(model.nhidden, model.nlayers, max_batch=batch, seed=seed * 100 + k + 1)
+        net.set_params(model.params[k])
+        losses = net.train(X, Y, batch=batch, epochs=epochs, lr0=lr0, seed=seed * 100 + k + 11,
+                           chunk=10, log=log)
+        model.params[k] = net.get_params()
+        curves.append(losses.tolist())
+        log("finetune: member %d, %d epochs, loss %.4g -> %.4g, %.1f s"
+            % (k, epochs, losses[0], losses[-1], time.time() - t0))
+        net.close()
+    model.meta["finetune"] = {"samples": int(X.shape[0]), "epochs": epochs, "lr0": lr0, "loss_curves": curves}
+    del X, Y
+    return model
+
+
 def trusted_cells(block_end, device):
     """Cells of the last RODAS3 step whose block neither failed nor accepted
     an error above FORCED_LIMIT (device bool tensor in cell order)."""

@@ -129,6 +129,12 @@ def main():
     ap.add_argument("--spinup", type=int, default=6)
     ap.add_argument("--eval-steps", type=int, default=24)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--finetune-from", default=None,
+                    help="fine-tune this model on its FP64 dataset plus cell-steps from its own rollouts")
+    ap.add_argument("--rollout-steps", type=int, default=24)
+    ap.add_argument("--rollout-fraction", type=float, default=0.05)
+    ap.add_argument("--finetune-epochs", type=int, default=50)
+    ap.add_argument("--finetune-lr", type=float, default=3e-4)
     args = ap.parse_args()
 
     import torch
@@ -140,8 +146,21 @@ def main():
     t0 = time.time()
     feat, targ = make_dataset(grid, seeds, args.steps, args.fraction, atol, rtol, log=log, skip=args.skip)
     log("dataset: %d cell-steps in %.1f s" % (feat.shape[0], time.time() - t0))
-    model = train_ensemble(feat, targ, nmembers=args.members, epochs=args.epochs, batch=args.batch,
-                           lr0=args.lr, log=log)
+    if args.finetune_from:
+        from brams_amd.surrogate import SurrogateModel, finetune_ensemble, make_rollout_dataset
+        model = SurrogateModel.load(args.finetune_from)
+        t1 = time.time()
+        rf, rt = make_rollout_dataset(model, grid, seeds, args.rollout_steps, args.rollout_fraction, atol, rtol,
+                                      spinup=args.spinup, log=log)
+        log("rollout dataset: %d cell-steps in %.1f s" % (rf.shape[0], time.time() - t1))
+        feat = torch.cat([feat, rf])
+        targ = torch.cat([targ, rt])
+        del rf, rt
+        model = finetune_ensemble(model, feat, targ, epochs=args.finetune_epochs, batch=args.batch,
+                                  lr0=args.finetune_lr, log=log)
+    else:
+        model = train_ensemble(feat, targ, nmembers=args.members, epochs=args.epochs, batch=args.batch,
+                               lr0=args.lr, log=log)
     ntol, ctol = calibrate_budgets(model, feat, targ)
     log("checks: N budget tol %.3g, C budget tol %.3g, spread tol %.3g" % (ntol, ctol, model.spread_tol))
     model.meta.update({"seeds": seeds, "steps": args.steps, "skip": args.skip, "fraction": args.fraction,
