@@ -94,3 +94,45 @@ def test_nested_grid_shapes_match_oracle(nx, nz, dt):
     assert st.attempts == ost.attempts and st.substeps == ost.substeps
     assert rep["nfail"] == 0, rep
     np.testing.assert_allclose(dt_gpu, hs.last_dt, rtol=1e-6)
+
+
+@pytest.mark.parametrize("method,ndyn", [("SEQUENTIAL", 1), ("PARALLEL", 1), ("PARALLEL", 2)])
+def test_split_methods_match_oracle(method, ndyn):
+    """The write-back forms of chem_spack_rodas3_dyndt.f90:713-798 through
+    the host entry: SEQUENTIAL overwrites sc_p; PARALLEL with N_DYN_CHEM=1
+    adds (new - old)/dtlt to sc_t and leaves sc_p; PARALLEL with N_DYN_CHEM>1
+    starts from sc_p - sc_t_dyn*N*dtlt and writes sc_t_dyn*N*dtlt + new."""
+    from brams_amd.chem import run_workload_host
+    print("case split %s N=%d, 16x16x6 seed 4" % (method, ndyn), flush=True)
+    grid = Grid(16, 16, 6)
+    wl = make_workload(grid, seed=4, sza="split", plume_sigma=3.0)
+    atol, rtol = tolerances("reference")
+    code = {"SEQUENTIAL": 1, "PARALLEL": 2}[method]
+    sc_t = sc_t_dyn = None
+    if method == "PARALLEL" and ndyn == 1:
+        rng = np.random.default_rng(4)
+        sc_t = [np.asfortranarray((rng.standard_normal(a.shape) * 1e-6).astype(np.float32)) for a in wl["sc_p"]]
+    if method == "PARALLEL" and ndyn > 1:
+        # a dynamics tendency that leaves a positive start state sc_p - sc_t_dyn*N*dtlt
+        sc_t_dyn = [np.asfortranarray((-0.01 * a / (ndyn * grid.dtlt)).astype(np.float32)) for a in wl["sc_p"]]
+    gpu, dt_gpu, rc, gpu_t = run_workload_host(wl, atol, rtol, split_method=method, n_dyn_chem=ndyn,
+                                               sc_t=sc_t, sc_t_dyn=sc_t_dyn)
+    hs, ost, _, orc = oracle_lib.run_oracle(wl, atol, rtol, split_method=code, n_dyn_chem=ndyn,
+                                            sc_t=sc_t, sc_t_dyn=sc_t_dyn)
+    assert rc == 0 and orc == 0
+    np.testing.assert_allclose(dt_gpu, hs.last_dt, rtol=1e-6)
+    if method == "PARALLEL" and ndyn == 1:
+        for s, (a, b) in enumerate(zip(gpu, wl["sc_p"])):      # sc_p untouched
+            assert np.array_equal(a, b), s
+        got, ref = gpu_t, hs.sc_t
+    else:
+        got, ref = gpu, hs.sc_p
+    worst = 0.0
+    for s, (a, b) in enumerate(zip(got, ref)):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        tol = 1e-6 * np.abs(b) + 1e-12 * max(np.abs(b).max(), 1e-30)
+        bad = np.abs(a - b) > tol
+        worst = max(worst, float((np.abs(a - b) / (np.abs(b) + 1e-30)).max()))
+        assert not bad.any(), (s, int(bad.sum()))
+    print("worst rel %.3g" % worst)
