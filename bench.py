@@ -118,6 +118,58 @@ def rodas3_flops(cells, cell_substeps, cell_attempts):
     return f_call * cells + f_sub * cell_substeps + f_att * cell_attempts
 
 
+def bench_nested(args):
+    """Three grids per step, each its own chem_rodas3_dyndt call (as the
+    reference drives nested grids): cell-steps/s over the 2.08M cells, every
+    step from the same seeded states (cold steps)."""
+    import torch
+    from brams_amd import abi
+    from brams_amd.chem import DeviceStep
+    from brams_amd.workload import Grid, make_workload, tolerances
+    atol, rtol = tolerances(args.tol)
+    grids = [Grid(200, 200, 40, dtlt=120.0), Grid(98, 98, 40, dtlt=60.0), Grid(50, 50, 40, dtlt=30.0)]
+    steps = [DeviceStep(make_workload(g, seed=0, sza="split", plume_sigma=20.0 * g.nx / 200.0,
+                                      plume_factor=50.0), atol, rtol) for g in grids]
+    ncell = sum(g.ncell for g in grids)
+    print("bench: nested grids, %d cells" % ncell, file=sys.stderr, flush=True)
+
+    def one_step():
+        out = []
+        for st in steps:
+            st.reset()
+            st.step()
+            s = abi.stats()
+            out.append((s.kernel_ms, s.cell_attempts, s.cells))
+        return out
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per = []
+    for i in range(args.steps):
+        per.append(one_step())
+        print("bench: nested step %d: %s ms" % (i, ", ".join("%.1f" % x[0] for x in per[-1])),
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    f_call, f_sub, f_att = flop_model()
+    line = {
+        "metric": "chem cell-steps/s, surrogate+fallback vs FP64 Rosenbrock, at <=1% rel err",
+        "value": ncell * args.steps / wall, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Philox generator, brams_amd/workload.py)",
+        "config": {"workload": "RELACS_TUV RODAS3 cold step on three nested grids: 200x200x40 (dt 120 s), "
+                               "98x98x40 (dt 60 s), 50x50x40 (dt 30 s), SZA split, plume, seed 0",
+                   "path": "plain FP64 RODAS3 (GPU), one call per grid", "tolerances": args.tol,
+                   "cells": ncell, "kernel_ms_per_grid": [sum(p[g][0] for p in per) / args.steps for g in range(3)],
+                   "attempts_per_cell_per_grid": [per[-1][g][1] / per[-1][g][2] for g in range(3)]},
+        "roofline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,7 +180,9 @@ def main():
     ap.add_argument("--ny", type=int, default=200)
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--tol", default="baseline", choices=["baseline", "reference"])
-    ap.add_argument("--path", default="surrogate", choices=["surrogate", "fp64"])
+    ap.add_argument("--path", default="surrogate", choices=["surrogate", "fp64", "nested"],
+                    help="nested: the plain FP64 step on three nested grids (200x200x40 at dt 120 s, "
+                         "98x98x40 at 60 s, 50x50x40 at 30 s), one call per grid")
     ap.add_argument("--model", default=os.path.join(ROOT, "brams_amd", "models", "relacs_tuv_surrogate.npz"))
     ap.add_argument("--spread-tol", type=float, default=None, help="override the model's spread check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -140,6 +194,9 @@ def main():
         print("bench.py measures one MI355X: the chemistry step has no cross-GPU exchange "
               "on one grid; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.path == "nested":
+        return bench_nested(args)
 
     import numpy as np
     import torch
@@ -261,7 +318,7 @@ def main():
             block_spread("last fp64 step")
     fp64_value = grid.ncell * K / wall_b
 
-    plain = {"kernel": "rodas3_blocks<64,false,false>", "bound": "fp64", "unit": "TFLOP/s",
+    plain = {"kernel": "rodas3_blocks<64,false,false,4>", "bound": "fp64", "unit": "TFLOP/s",
              "avg_ms": b_main_ms / K, "achieved": b_flops_main / (b_main_ms * 1e-3) / 1e12,
              "peak": FP64_PEAK_TFLOPS, "traffic": RODAS3_BYTES_PER_CELL_ATTEMPT * b_att_main / K,
              "algorithmic_bytes": 528 * grid.ncell}
@@ -277,7 +334,7 @@ def main():
         fb_flops = sum(rodas3_flops(s.fallback_cells, s.fallback_cell_substeps, s.fallback_cell_attempts)
                        for s in sstats)
         fb_att = sum(s.fallback_cell_attempts for s in sstats)
-        fallback = {"kernel": "rodas3_blocks<64,false,true>", "bound": "fp64", "unit": "TFLOP/s",
+        fallback = {"kernel": "rodas3_blocks<64,false,true,1>", "bound": "fp64", "unit": "TFLOP/s",
                     "avg_ms": fb_ms / K, "achieved": fb_flops / max(fb_ms * 1e-3, 1e-9) / 1e12,
                     "peak": FP64_PEAK_TFLOPS, "traffic": RODAS3_LIST_BYTES_PER_CELL_ATTEMPT * fb_att / K,
                     "algorithmic_bytes": 528 * sum(s.fallback_cells for s in sstats) / K}

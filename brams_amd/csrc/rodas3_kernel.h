@@ -86,22 +86,23 @@ __device__ __forceinline__ void block_reduce(double& errmax, int& rej, int& fail
 // blocks.  Otherwise the mask is read from p.mask.
 // A lane's state (about 1000 doubles: LU, rate constants, stage vectors)
 // never fits the register file, so the compiler spills whatever is asked
-// of it; what matters is how many waves hide the spill traffic.  Measured on
-// the 1.6M-cell workload (25 000 blocks; cold step / warm step, ms) with the
-// register budget set by the waves per SIMD: 1: 1294 / 120, 2: 1059 / 102,
-// 3: 998 / 98, 4: 939 / 95, 6: 1023 / 112, 8: 1082 / 118; the scratch per
-// lane grows only from 8.0 to 9.2 KB over that range.  A launch of a few
-// thousand blocks (the surrogate's fallback) has too few waves to use the
-// occupancy and is faster with the whole register file (31 against 44 ms
-// for 2300 blocks), so the list form keeps one wave per SIMD.
-constexpr int kRodas3WavesPerSimd = 4;
-constexpr int kRodas3ListWavesPerSimd = 1;
+// of it; what matters is how many waves hide the spill traffic.  WPS sets
+// the register budget by the waves per SIMD it must allow.  Measured on the
+// 1.6M-cell workload (25 000 blocks; cold step / warm step, ms): WPS 1:
+// 1294 / 120, 2: 1059 / 102, 3: 998 / 98, 4: 939 / 95, 6: 1023 / 112,
+// 8: 1082 / 118, the scratch per lane growing only from 8.0 to 9.2 KB.  A
+// launch of a few thousand blocks has too few waves to use the occupancy
+// and runs faster with the whole register file, so the launch picks WPS by
+// its block count (rodas3_launch.hip).  Crossover measurements (WPS 4 / 1,
+// ms): 6002 blocks (98x98x40) 255 / 285; 2300 blocks (the surrogate's
+// fallback) 44 / 31; 1562 blocks (50x50x40) 93 / 74.
+constexpr int kRodas3ManyWaves = 4;        // WPS of large launches
+constexpr int kRodas3ManyBlocks = 4096;    // from this many blocks on
 
 // LIST: the blocks are consecutive WG-sized runs of p.cell_list (fallback of
 // the surrogate step) instead of the AllocIndex partition.
-template <int WG, bool DYN, bool LIST = false>
-__global__ __launch_bounds__(WG, LIST ? kRodas3ListWavesPerSimd : kRodas3WavesPerSimd) void rodas3_blocks(
-    StepParams p, int block0) {
+template <int WG, bool DYN, bool LIST, int WPS>
+__global__ __launch_bounds__(WG, WPS) void rodas3_blocks(StepParams p, int block0) {
   const int b = (!LIST && p.block_order) ? p.block_order[blockIdx.x] : block0 + blockIdx.x;
   int start, ncell;
   if constexpr (LIST) {                             // fallback blocks of listed cells
