@@ -382,7 +382,8 @@ int brams_sur_normalize(const float* raw, int64_t n, int ncol, const float* mu, 
                         void* out, int ldo, int out_bf16) {
   if (!raw || !out || n <= 0 || ncol > ldo) return fail(-2, "bad normalize arguments");
   const long long tot = (long long)n * ldo;
-  const unsigned blocks = (unsigned)((tot + 255) / 256);
+  // at most 2^20 workgroups of 256: the grid stays far below 2^32 work-items
+  const unsigned blocks = (unsigned)std::min<long long>((tot + 255) / 256, 1LL << 20);
   if (out_bf16)
     hipLaunchKernelGGL(brams::normalize_rows<true>, dim3(blocks), dim3(256), 0, nullptr, raw,
                        (long long)n, ncol, mu, sd, out, ldo);

@@ -103,19 +103,23 @@ __global__ __launch_bounds__(256) void column_moments(const float* X, long long 
   }
 }
 
-// z-score columns into a padded row: bf16 (features) or f32 (targets)
+// z-score columns into a padded row: bf16 (features) or f32 (targets).
+// Grid-stride: a dispatch's size in work-items is a 32-bit quantity, and a
+// dataset of 37.7M rows x 128 columns (4.8e9 elements) would exceed it.
 template <bool BF16>
 __global__ void normalize_rows(const float* raw, long long n, int ncol, const float* mu,
                                const float* sd, void* out, int ldo) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * ldo) return;
-  const long long r = i / ldo;
-  const int c = (int)(i % ldo);
-  const float v = c < ncol ? (raw[r * ncol + c] - mu[c]) / sd[c] : 0.0f;
-  if constexpr (BF16)
-    static_cast<__bf16*>(out)[i] = (__bf16)v;
-  else
-    static_cast<float*>(out)[i] = v;
+  const long long tot = n * ldo;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long r = i / ldo;
+    const int c = (int)(i % ldo);
+    const float v = c < ncol ? (raw[r * ncol + c] - mu[c]) / sd[c] : 0.0f;
+    if constexpr (BF16)
+      static_cast<__bf16*>(out)[i] = (__bf16)v;
+    else
+      static_cast<float*>(out)[i] = v;
+  }
 }
 
 // normalised features straight from the state (inference): X[g][128] bf16
