@@ -115,13 +115,13 @@ def rollout(model, grid, steps, atol, rtol, seed=0, sza_step=0.5, spinup=0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--seeds", default="1,2,3,4,5,6,7,8,9,10")
-    ap.add_argument("--steps", type=int, default=12)
-    ap.add_argument("--fraction", type=float, default=0.125)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--fraction", type=float, default=0.10)
     ap.add_argument("--nx", type=int, default=200)
     ap.add_argument("--ny", type=int, default=200)
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--members", type=int, default=3)
-    ap.add_argument("--epochs", type=int, default=150)
+    ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=os.path.join(ROOT, "brams_amd", "models", "relacs_tuv_surrogate.npz"))
