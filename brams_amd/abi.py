@@ -20,7 +20,8 @@ _ip = C.POINTER(C.c_int32)
 # symbols include/brams_chem.h declares
 EXPORTS = ["brams_chem_init", "brams_chem_rodas3_dyndt", "brams_chem_rodas3_dyndt_device",
            "brams_chem_alloc_index", "brams_chem_get_stats", "brams_chem_last_error",
-           "brams_chem_finalize", "brams_chem_block_status"]
+           "brams_chem_finalize", "brams_chem_block_status", "brams_chem_block_thresholds",
+           "brams_chem_set_form", "brams_chem_get_tuning"]
 
 
 class ChemArgs(C.Structure):
@@ -70,7 +71,9 @@ class ChemStats(C.Structure):
                 ("mask_nnz", C.c_int32), ("kernel_ms", C.c_float),
                 ("cell_substeps", C.c_int64), ("cell_attempts", C.c_int64),
                 ("block1_ms", C.c_float), ("main_ms", C.c_float), ("main_cells", C.c_int64),
-                ("forced_blocks", C.c_int64), ("max_forced_err", C.c_float)]
+                ("forced_blocks", C.c_int64), ("max_forced_err", C.c_float),
+                ("threshold_pivots", C.c_int64), ("order_threshold_pivots", C.c_int64),
+                ("kernel_form", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -107,6 +110,9 @@ def lib():
         L.brams_chem_last_error.restype = C.c_char_p
         L.brams_chem_finalize.argtypes = []
         L.brams_chem_block_status.argtypes = [_ip, _ip, _ip, C.POINTER(C.c_float), C.c_int32]
+        L.brams_chem_block_thresholds.argtypes = [_ip, C.c_int32]
+        L.brams_chem_set_form.argtypes = [C.c_int32]
+        L.brams_chem_get_tuning.argtypes = [_ip, C.POINTER(C.c_float), C.c_int32]
         _lib = L
     return _lib
 
@@ -127,6 +133,29 @@ def block_status(nob):
                                         f.ctypes.data_as(_ip), e.ctypes.data_as(C.POINTER(C.c_float)),
                                         nob), "brams_chem_block_status")
     return a, s, f, e
+
+
+def block_thresholds(nob):
+    """Per-block count of pivots failing Sparse1.3's threshold rule, last step."""
+    import numpy as np
+    c = np.zeros(nob, np.int32)
+    check(lib().brams_chem_block_thresholds(c.ctypes.data_as(_ip), nob), "brams_chem_block_thresholds")
+    return c
+
+
+def set_form(form):
+    """Kernel form of the 64-cell blocks (-1: autotune per grid); see brams_chem_set_form."""
+    check(lib().brams_chem_set_form(form), "brams_chem_set_form")
+
+
+def tuning():
+    """(chosen form of the current grid, autotuner timings in ms per form)."""
+    import numpy as np
+    f = np.zeros(1, np.int32)
+    ms = np.zeros(8, np.float32)
+    n = check(lib().brams_chem_get_tuning(f.ctypes.data_as(_ip), ms.ctypes.data_as(C.POINTER(C.c_float)), 8),
+              "brams_chem_get_tuning")
+    return int(f[0]), ms[:n]
 
 
 def stats():

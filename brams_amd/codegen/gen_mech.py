@@ -254,10 +254,14 @@ def main():
     w("")
     # ---------------- factor
     w("// spFactor.c:336-401 on the fixed pivot order; lu[] keeps L (reciprocal")
-    w("// diagonal) and unit-diagonal U.  Returns nonzero on a zero pivot.")
+    w("// diagonal) and unit-diagonal U.  Returns nonzero on a zero pivot.  *viol")
+    w("// is increased by the pivots that fail Sparse1.3's threshold rule, which")
+    w("// accepts a pivot only if RelThreshold (1e-3, spAllocate.c:191) times the")
+    w("// largest other entry of its reduced column is below |pivot|")
+    w("// (spFactor.c:202-203, FindBiggestInColExclude spFactor.c:2019-2050).")
     w("__device__ __forceinline__ int mech_factor(const double* rk, const double* y, double igdt,")
-    w("                                           const unsigned* mw, double* lu) {")
-    w("  int bad = 0;")
+    w("                                           const unsigned* mw, double* lu, int* viol) {")
+    w("  int bad = 0, nv = 0;")
     for J in range(ns):
         rows = [I for I in range(ns) if F[I][J]]
         w("  { // column %d (species %d)" % (J, perm[J]))
@@ -274,11 +278,19 @@ def main():
                 if F[I][K]:
                     w("      d%d -= u * lu[%d];" % (I, slot[(I, K)]))
             w("    }")
-        for I in rows:
-            if I > J:
-                w("    lu[%d] = d%d;" % (slot[(I, J)], I))
+        below = [I for I in rows if I > J]
+        for I in below:
+            w("    lu[%d] = d%d;" % (slot[(I, J)], I))
+        if below:
+            w("    { double cm = fabs(d%d);" % below[0])
+            for I in below[1:]:
+                w("      cm = fmax(cm, fabs(d%d));" % I)
+            w("      nv += (cm * 1.0e-3 >= fabs(d%d)) ? 1 : 0; }" % J)
+        else:
+            w("    nv += (d%d == 0.0) ? 1 : 0;" % J)
         w("    bad |= (d%d == 0.0); lu[%d] = 1.0 / d%d;" % (J, slot[(J, J)], J))
         w("  }")
+    w("  *viol += nv;")
     w("  return bad;")
     w("}")
     w("")
@@ -358,12 +370,314 @@ def main():
     with open(os.path.join(os.path.dirname(OUT), "relacs_tuv_flops.json"), "w") as fh:
         json.dump(flops, fh, indent=1)
     text = "\n".join(L) + "\n"
+    emit_slab(m, ns, react, prod, jt, perm, F, slot, sentries, sidx, wexpr, sum_terms, jexpr, rate_expr)
     out = sys.argv[1] if len(sys.argv) > 1 else OUT
     old = open(out).read() if os.path.exists(out) else None
     if old != text:
         with open(out, "w") as fh:
             fh.write(text)
     print("generated %s: nstruct=%d nlu=%d" % (out, len(sentries), nlu))
+
+
+SLAB_OUT = os.path.join(PKG, "csrc", "generated", "relacs_tuv_slab.h")
+RL_REGS = 120        # factorisation entries kept in registers at most
+
+
+def emit_slab(m, ns, react, prod, jt, perm, F, slot, sentries, sidx, wexpr, sum_terms, jexpr, rate_expr):
+    """Device functions of the slab kernel (rodas3_slab.h), templated over
+    memory accessors: an accessor is anything with operator[](int) returning
+    the lane's element, so the same generated code runs on registers, LDS and
+    global scratch.
+
+      mech_kinetic_t  rk[128] written through an accessor
+      mech_fun_t      f = S w(rk, y), each rate constant loaded once
+      mech_jac_t      the Jacobian entries J[e] (e in mech_jac order), column
+                      by column with the column's rate constants and
+                      concentrations loaded once
+      mech_factor_rl  factorisation of the masked 1/(gamma h) I - J read from
+                      J[e], right-looking with lazy assembly: every entry is
+                      a register from its first update to its final value,
+                      which is written once to the LU accessor; at most 169
+                      entries are live at a time.  Each entry receives its
+                      updates in ascending pivot order and U(K,J) =
+                      A(K,J) * (1/pivot_K), the operations of mech_factor in
+                      the same order, so the factors are bit-identical.
+      mech_solve_t    mech_solve reading the LU through an accessor
+    """
+    L = []
+    w = L.append
+    jorder = sorted(jt.keys(), key=lambda e: (e[1], e[0]))          # column-major
+    jidx = {e: k for k, e in enumerate(jorder)}
+    nlu = len(slot)
+    w("// Generated by brams_amd/codegen/gen_mech.py from brams_amd/mechanism/relacs_tuv.json.")
+    w("// Accessor-templated device functions of the slab kernel (rodas3_slab.h).")
+    w("#pragma once")
+    w("#include \"relacs_tuv_gen.h\"")
+    w("#define MECH_NJAC %d" % len(jorder))
+    w("")
+    # ---- kinetic
+    w("// chem_spack_kinetic.f90:7-435")
+    w("template <class RK>")
+    w("__device__ __forceinline__ void mech_kinetic_t(double T, double P, double vapp, const double* J, RK& rk) {")
+    w("  const double M = P * 7.243e16 / T;")
+    w("  const double H2O = 29.0 * M * vapp / (18.0 + 11.0 * vapp);")
+    w("  const double lnT = log(T);")
+    w("  (void)H2O; (void)lnT;")
+    for r, rx in enumerate(m["reactions"]):
+        law = rx["rate"]
+        e = rate_expr(law["family"], law["params"], r)
+        post = law["post"]
+        if post:
+            kind = post[0][0]
+            if kind == "mul_M_const":
+                w("  rk[%d] = (%s) * M * %s;" % (r, e, lit(post[0][1])))
+            elif kind == "mul_H2O":
+                w("  rk[%d] = (%s) * H2O;" % (r, e))
+            elif kind == "mul_troe":
+                w("  rk[%d] = rl_troe(T, M, %s) * (%s);" % (r, ", ".join(lit(x) for x in post[0][1:]), e))
+            else:
+                raise ValueError(kind)
+        else:
+            w("  rk[%d] = %s;" % (r, e))
+    w("}")
+    w("")
+    # ---- fun, reaction-major: each rate w_r is added to the sums of its
+    # species as soon as it is formed, so only the 47 sums are live.  Every
+    # sum still receives its terms in ascending reaction order with the same
+    # coefficients, the evaluation order of mech_fun.
+    used = sorted({r for terms in prod.values() for _, r in terms})
+    by_r = {}
+    for sp in range(ns):
+        for k, (c, r) in enumerate(prod.get(sp, [])):
+            by_r.setdefault(r, []).append((sp, c, k == 0))
+    w("// chem_spack_rates.f90:63-192 and chem_spack_fexchem.f90:73-676")
+    w("template <class RK, class Y, class FO>")
+    w("__device__ __forceinline__ void mech_fun_t(const RK& rk, const Y& y, FO& f) {")
+    for sp in range(ns):
+        if prod.get(sp):
+            w("  double f%d;" % sp)
+    for r in used:
+        w("  { const double w = %s;" % wexpr(r))
+        for sp, c, first in by_r[r]:
+            if first:
+                if c == 1.0:
+                    w("    f%d = w;" % sp)
+                elif c == -1.0:
+                    w("    f%d = -w;" % sp)
+                else:
+                    w("    f%d = %s * w;" % (sp, lit(c)))
+            else:
+                if c == 1.0:
+                    w("    f%d = f%d + w;" % (sp, sp))
+                elif c == -1.0:
+                    w("    f%d = f%d - w;" % (sp, sp))
+                elif c < 0:
+                    w("    f%d = f%d - %s * w;" % (sp, sp, lit(-c)))
+                else:
+                    w("    f%d = f%d + %s * w;" % (sp, sp, lit(c)))
+        w("  }")
+    for sp in range(ns):
+        w("  f[%d] = %s;" % (sp, ("f%d" % sp) if prod.get(sp) else "0.0"))
+    w("}")
+    w("")
+    # ---- jacobian entries, column by column
+    w("// chem_spack_dratedc.f90 + chem_spack_jacdchemdc.f90: entry e of the")
+    w("// column-major list mech_jac_i/j (0-based species) is dF_i/dy_j")
+    w("static const int mech_jac_i_host[MECH_NJAC] = {%s};" % ", ".join(str(i) for i, _ in jorder))
+    w("static const int mech_jac_j_host[MECH_NJAC] = {%s};" % ", ".join(str(j) for _, j in jorder))
+    w("// The call's value-based mask (rodas3:470-482) is applied here, once per")
+    w("// sub-step: an off-diagonal entry outside the mask is stored as -0.0, so")
+    w("// that its matrix entry -J is the +0.0 of the plain kernel; diagonal")
+    w("// entries are stored unmasked (their mask bit is applied per attempt,")
+    w("// with 1/(gamma h)).")
+    w("template <class RK, class Y, class JO>")
+    w("__device__ __forceinline__ void mech_jac_t(const RK& rk, const Y& y, const unsigned* mw, JO& jo) {")
+    col = None
+    for (i, j) in jorder:
+        if j != col:
+            if col is not None:
+                w("  }")
+            col = j
+            rs = sorted({r for (ii, jj) in jorder if jj == j for _, r, _ in jt[(ii, jj)]})
+            ys = sorted({o for (ii, jj) in jorder if jj == j for _, _, o in jt[(ii, jj)] if o >= 0})
+            w("  { // column %d" % j)
+            for r in rs:
+                w("    const double k%d = rk[%d];" % (r, r))
+            for o in ys:
+                w("    const double c%d = y[%d];" % (o, o))
+        terms = []
+        for c, r, o in jt[(i, j)]:
+            ref = "(k%d * c%d)" % (r, o) if o >= 0 else "k%d" % r
+            terms.append((c, ref))
+        if i == j:
+            w("    jo[%d] = %s;" % (jidx[(i, j)], sum_terms(terms)))
+        else:
+            e = sidx[(i, j)]
+            w("    jo[%d] = ((mw[%d] >> %d) & 1u) ? (%s) : -0.0;" % (jidx[(i, j)], e // 32, e % 32,
+                                                             sum_terms(terms)))
+    w("  }")
+    w("}")
+    w("")
+    # ---- value-based mask of one cell (rodas3:470-482) from its Jacobian
+    # entries: an off-diagonal entry is kept where -J != 0, a diagonal one
+    # where 1/(gamma h) - J_ii != 0 (the values mech_mask tests)
+    w("// rodas3:470-482 from stored Jacobian entries: the off-diagonal bits")
+    w("// (constant over a sub-step) and the diagonal bits (one per attempt)")
+    w("template <class JR>")
+    w("__device__ __forceinline__ void mech_mask_offdiag(const JR& jac, unsigned* mw) {")
+    w("  unsigned acc[MECH_MASK_WORDS] = {0};")
+    for e, (i, j) in enumerate(sentries):
+        if i != j:
+            w("  acc[%d] |= (jac[%d] != 0.0 ? 1u : 0u) << %d;" % (e // 32, jidx[(i, j)], e % 32))
+    w("  for (int k = 0; k < MECH_MASK_WORDS; ++k) mw[k] = acc[k];")
+    w("}")
+    w("template <class JR>")
+    w("__device__ __forceinline__ void mech_mask_diag(const JR& jac, double igdt, unsigned* mw) {")
+    w("  unsigned acc[MECH_MASK_WORDS] = {0};")
+    for e, (i, j) in enumerate(sentries):
+        if i == j:
+            v = "igdt - jac[%d]" % jidx[(i, i)] if (i, i) in jt else "igdt"
+            w("  acc[%d] |= ((%s) != 0.0 ? 1u : 0u) << %d;" % (e // 32, v, e % 32))
+    w("  for (int k = 0; k < MECH_MASK_WORDS; ++k) mw[k] = acc[k];")
+    w("}")
+    w("")
+    # ---- right-looking lazy factorisation
+    Lcol = [[I for I in range(K + 1, ns) if F[I][K]] for K in range(ns)]
+    Urow = [[J for J in range(K + 1, ns) if F[K][J]] for K in range(ns)]
+    birth = {}
+    for (I, J) in slot:
+        k0 = min(I, J)
+        for K in range(min(I, J)):
+            if F[I][K] and F[K][J]:
+                k0 = K
+                break
+        birth.setdefault(k0, []).append((I, J))
+
+    def mbit(e):
+        return "((mw[%d] >> %d) & 1u)" % (e // 32, e % 32)
+
+    # Statements in emission order; each names the entries it reads or
+    # writes, so that registers can be budgeted: when more than RL_REGS
+    # entries are live, the one whose next use is farthest (Belady) is parked
+    # in its own LU slot (the factorisation never reads LU slots, and the
+    # slot's final value is written later) and reloaded before its next use.
+    stmts = []          # (kind, text, entries used, entries born, entries dying)
+    for K in range(ns):
+        stmts.append(("code", "  // step %d (species %d)" % (K, perm[K]), [], [], []))
+        for (I, J) in sorted(birth.get(K, [])):
+            i, j = perm[I], perm[J]
+            # diagonal mask bits as a 0/1 factor: with the bit set the value
+            # is exact; a cleared bit gives a zero pivot either way, which
+            # ends the block at its first attempt as in mech_factor
+            if (i, j) in jt and i == j:
+                txt = "  double a%d_%d = (igdt - jac[%d]) * (double)%s;" % (I, J, jidx[(i, j)], mbit(sidx[(i, j)]))
+            elif (i, j) in jt:
+                txt = "  double a%d_%d = -jac[%d];" % (I, J, jidx[(i, j)])
+            elif i == j:
+                txt = "  double a%d_%d = igdt * (double)%s;" % (I, J, mbit(sidx[(i, j)]))
+            else:
+                txt = "  double a%d_%d = 0.0;" % (I, J)
+            stmts.append(("code", txt, [], [(I, J)], []))
+        below = Lcol[K]
+        for I in below:
+            stmts.append(("code", "  lu[%d] = a%d_%d;" % (slot[(I, K)], I, K), [(I, K)], [], []))
+        if below:
+            stmts.append(("code", "  { double cm = fabs(a%d_%d);" % (below[0], K), [(below[0], K)], [], []))
+            for I in below[1:]:
+                stmts.append(("code", "    cm = fmax(cm, fabs(a%d_%d));" % (I, K), [(I, K)], [], []))
+            stmts.append(("code", "    nv += (cm * 1.0e-3 >= fabs(a%d_%d)) ? 1 : 0; }" % (K, K), [(K, K)], [], []))
+        else:
+            stmts.append(("code", "  nv += (a%d_%d == 0.0) ? 1 : 0;" % (K, K), [(K, K)], [], []))
+        stmts.append(("code", "  bad |= (a%d_%d == 0.0);" % (K, K), [(K, K)], [], []))
+        stmts.append(("code", "  const double r%d = 1.0 / a%d_%d; lu[%d] = r%d;" % (K, K, K, slot[(K, K)], K),
+                      [(K, K)], [], [(K, K)]))
+        for J in Urow[K]:
+            stmts.append(("code", "  { const double u = a%d_%d * r%d; lu[%d] = u;" % (K, J, K, slot[(K, J)]),
+                          [(K, J)], [], [(K, J)]))
+            for I in below:
+                if F[I][J]:
+                    stmts.append(("code", "    a%d_%d -= u * a%d_%d;" % (I, J, I, K), [(I, J), (I, K)], [], []))
+            stmts.append(("code", "  }", [], [], []))
+        stmts.append(("die", "", [], [], [(I, K) for I in below]))
+    # next-use tables
+    uses = {}
+    for n, (_, _, used_e, born_e, _) in enumerate(stmts):
+        for e in used_e + born_e:
+            uses.setdefault(e, []).append(n)
+    ptr = {e: 0 for e in uses}
+
+    def next_use(e, n):
+        lst, k = uses[e], ptr[e]
+        while k < len(lst) and lst[k] <= n:
+            k += 1
+        ptr[e] = k
+        return lst[k] if k < len(lst) else 1 << 30
+
+    live, parked, nload, nstore = set(), set(), 0, 0
+    body = []
+    for n, (kind, txt, used_e, born_e, dying) in enumerate(stmts):
+        if kind == "die":
+            for e in dying:
+                live.discard(e)
+            continue
+        for e in used_e:
+            if e in parked:
+                parked.discard(e)
+                body.append("  a%d_%d = lu[%d];" % (e[0], e[1], slot[e]))
+                nload += 1
+            live.add(e)
+        for e in born_e:
+            live.add(e)
+        while len(live) > RL_REGS:
+            cands = [x for x in live if x not in used_e and x not in born_e]
+            victim = max(cands, key=lambda x: next_use(x, n))
+            live.discard(victim)
+            parked.add(victim)
+            body.append("  lu[%d] = a%d_%d;" % (slot[victim], victim[0], victim[1]))
+            nstore += 1
+        body.append(txt)
+        for e in dying:
+            live.discard(e)
+
+    w("// spFactor.c:336-401 arithmetic, right-looking with lazy assembly: A(I,J) =")
+    w("// masked 1/(gamma h) I - J in pivot order; lu[] gets L (reciprocal pivots)")
+    w("// and unit-diagonal U.  At most %d entries are kept in registers; the" % RL_REGS)
+    w("// others wait in their own LU slot (%d parkings, %d reloads per call)." % (nstore, nload))
+    w("// *viol counts pivots failing Sparse1.3's threshold rule")
+    w("// (spFactor.c:202-203).  Returns nonzero on a zero pivot.")
+    w("#define MECH_RL_PARK_STORES %d" % nstore)
+    w("#define MECH_RL_PARK_LOADS %d" % nload)
+    w("template <class JR, class LU>")
+    w("__device__ __forceinline__ int mech_factor_rl(const JR& jac, double igdt, const unsigned* mw, LU& lu, int* viol) {")
+    w("  int bad = 0, nv = 0;")
+    L.extend(body)
+    w("  *viol += nv;")
+    w("  return bad;")
+    w("}")
+    w("")
+    # ---- solve
+    w("// spSolve.c:183-218: x (species order) overwritten by the solution")
+    w("template <class LU, class X>")
+    w("__device__ __forceinline__ void mech_solve_t(const LU& lu, X& x) {")
+    for I in range(ns):
+        w("  double c%d = x[%d];" % (I, perm[I]))
+    for I in range(ns):
+        w("  c%d *= lu[%d];" % (I, slot[(I, I)]))
+        for R in range(I + 1, ns):
+            if F[R][I]:
+                w("  c%d -= c%d * lu[%d];" % (R, I, slot[(R, I)]))
+    for I in range(ns - 1, -1, -1):
+        for Jc in range(I + 1, ns):
+            if F[I][Jc]:
+                w("  c%d -= lu[%d] * c%d;" % (I, slot[(I, Jc)], Jc))
+    for I in range(ns):
+        w("  x[%d] = c%d;" % (perm[I], I))
+    w("}")
+    text = "\n".join(L) + "\n"
+    old = open(SLAB_OUT).read() if os.path.exists(SLAB_OUT) else None
+    if old != text:
+        with open(SLAB_OUT, "w") as fh:
+            fh.write(text)
 
 
 if __name__ == "__main__":

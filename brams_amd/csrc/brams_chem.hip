@@ -2,7 +2,7 @@
 //
 // Host side of the drop-in for chem_rodas3_dyndt
 // (src/brams/ccatt/chem_spack_rodas3_dyndt.f90:41-131): validates the
-// arguments, keeps persistent device buffers per grid size, moves host arrays
+// arguments, keeps persistent device buffers per grid, moves host arrays
 // to HBM and back for the Fortran entry point, and launches the kernels of
 // rodas3_kernel.h:
 //   1. block 1 alone (its value-based sparsity mask changes every attempt and
@@ -23,6 +23,7 @@
 #include "../../include/brams_surrogate.h"
 #include "rodas3_kernel.h"
 #include "rodas3_launch.h"
+#include "rodas3_slab_launch.h"
 #include "sched_kernels.h"
 #include "sur_step_kernels.h"
 
@@ -68,27 +69,62 @@ struct DevBuf {
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Kernel forms of the 64-cell partition blocks: the slab layouts
+// 0..kSlabVariants-1 (rodas3_slab.h) and the per-lane kernel
+// (rodas3_kernel.h).  All give bit-identical results.
+constexpr int kFormPerLane = brams::kSlabVariants;
+constexpr int kForms = brams::kSlabVariants + 1;
+// The slab forms are switched off: on gfx950 their kernels did not
+// terminate, even for a one-block grid with the attempts capped at 300,
+// while a host build of the same block integration matches the oracle bit
+// for bit.  Until that is understood every call takes the per-lane form.
+constexpr bool kSlabEnabled = false;
+
+// Device state of one grid, created on the first call for that grid and
+// kept until brams_chem_finalize (the reference allocates its block index
+// per grid on the first chemistry call, chemistry.f90:310-320).  Grids are
+// told apart by their extents and interior bounds, so nested grids called in
+// turn keep their own partition, block history and fallback memory.
+struct GridState {
+  int m1 = 0, m2 = 0, m3 = 0, ia = 0, iz = 0, ja = 0, jz = 0;
+  long long ncell = 0;
+  std::vector<int> sizes;        // AllocIndex block sizes
+  DevBuf block_start, mask, attempts, substeps, failed, forced, thresh, order_thresh;
+  bool blk_hist = false;         // attempts[] holds a previous call on this grid
+  long long hist_cells = -1;     // fallback memory of the surrogate step (-1: to be started)
+  DevBuf cell_dt;
+  std::vector<int> h_att, h_sub, h_fail, h_thresh;
+  std::vector<float> h_forced;
+  int form = -1;                 // kernel form of the 64-cell blocks (-1: not tuned yet)
+  float tune_ms[kForms] = {0, 0, 0, 0};
+  bool same(int a1, int a2, int a3, int b1, int b2, int b3, int b4) const {
+    return m1 == a1 && m2 == a2 && m3 == a3 && ia == b1 && iz == b2 && ja == b3 && jz == b4;
+  }
+};
+
 struct State {
   bool init = false;
   int device = 0;
-  // persistent per-call buffers
-  DevBuf block_start, mask, attempts, substeps, failed, forced;
+  std::vector<GridState*> grids;
+  GridState* cur = nullptr;      // grid of the last call
   // host-entry staging
   DevBuf sc_p, sc_t, sc_t_dyn, met, jphoto, last_dt, consts;
   // surrogate step
   DevBuf sur_X, sur_out, sur_flag, sur_list, sur_chunk, sur_reason, fb_dt, fb_att, mask_all;
-  // work ordering (sched_kernels.h): sort scratch, block order, and the
-  // fallback's per-cell step memory
-  DevBuf sort_keys, sort_ids, sort_count, sort_off, order, cell_dt;
-  bool blk_hist = false;         // attempts[] holds a previous call on this partition
-  long long hist_cells = -1;     // cells of the fallback memory (-1: to be started)
+  // work ordering (sched_kernels.h): sort scratch and block order
+  DevBuf sort_keys, sort_ids, sort_count, sort_off, order;
+  // slab kernels: per-slot scratch regions and the work-queue counter
+  DevBuf slab_lu, slab_jac, slab_rk, slab_vec, qcount;
+  long long slab_slots = 0;
+  int n_cu = 0;
+  int force_form = -1;           // brams_chem_set_form (-1: autotune per grid)
+  // autotuning scratch (copies of the state a step writes)
+  DevBuf tune_sc, tune_dt;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evb = nullptr, evm = nullptr;
+  hipEvent_t sev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   long long main_cells = 0;
+  int last_form = -1;
   brams_chem_stats stats{};
-  std::vector<int> part_sizes;   // block sizes of the cached partition
-  long long part_ncell = -1;
-  std::vector<int> h_att, h_sub, h_fail;
-  std::vector<float> h_forced;
 };
 State g;
 
@@ -106,23 +142,44 @@ int partition(int block_size, long long ncell, std::vector<int>& sizes) {
   return nob;
 }
 
-int ensure_partition(long long ncell) {
-  if (g.part_ncell == ncell) return 0;
-  int nob = partition(64, ncell, g.part_sizes);
+// the state of the grid with these extents, created (partition, device
+// bookkeeping) on its first call
+int ensure_grid(int m1, int m2, int m3, int ia, int iz, int ja, int jz) {
+  for (GridState* gs : g.grids)
+    if (gs->same(m1, m2, m3, ia, iz, ja, jz)) {
+      g.cur = gs;
+      return 0;
+    }
+  const long long ncell = (long long)(m1 - 2) * (iz - ia + 1) * (jz - ja + 1);
+  std::vector<int> sizes;
+  int nob = partition(64, ncell, sizes);
   if (nob < 0) return fail(-2, "empty grid");
-  for (int s : g.part_sizes)
-    if (s > 128) return fail(-3, "block of %d cells exceeds 128", s);
+  for (int sz : sizes)
+    if (sz > 128) return fail(-3, "block of %d cells exceeds 128", sz);
+  GridState* gs = new GridState;
+  gs->m1 = m1; gs->m2 = m2; gs->m3 = m3; gs->ia = ia; gs->iz = iz; gs->ja = ja; gs->jz = jz;
+  gs->ncell = ncell;
+  gs->sizes = sizes;
   std::vector<int> start(nob + 1, 0);
-  for (int b = 0; b < nob; ++b) start[b + 1] = start[b] + g.part_sizes[b];
-  HIP_TRY(g.block_start.ensure(sizeof(int) * (nob + 1)));
-  HIP_TRY(hipMemcpy(g.block_start.p, start.data(), sizeof(int) * (nob + 1), hipMemcpyHostToDevice));
-  HIP_TRY(g.attempts.ensure(sizeof(int) * nob));
-  HIP_TRY(g.substeps.ensure(sizeof(int) * nob));
-  HIP_TRY(g.failed.ensure(sizeof(int) * nob));
-  HIP_TRY(g.forced.ensure(sizeof(float) * nob));
-  g.part_ncell = ncell;
-  g.blk_hist = false;
-  g.hist_cells = -1;
+  for (int b = 0; b < nob; ++b) start[b + 1] = start[b] + sizes[b];
+  hipError_t e = gs->block_start.ensure(sizeof(int) * (nob + 1));
+  if (e == hipSuccess) e = hipMemcpy(gs->block_start.p, start.data(), sizeof(int) * (nob + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = gs->mask.ensure(sizeof(unsigned) * MECH_MASK_WORDS);
+  if (e == hipSuccess) e = gs->attempts.ensure(sizeof(int) * nob);
+  if (e == hipSuccess) e = gs->substeps.ensure(sizeof(int) * nob);
+  if (e == hipSuccess) e = gs->failed.ensure(sizeof(int) * nob);
+  if (e == hipSuccess) e = gs->forced.ensure(sizeof(float) * nob);
+  if (e == hipSuccess) e = gs->thresh.ensure(sizeof(int) * nob);
+  if (e == hipSuccess) e = gs->order_thresh.ensure(sizeof(int));
+  if (e != hipSuccess) {
+    DevBuf* bufs[] = {&gs->block_start, &gs->mask, &gs->attempts, &gs->substeps, &gs->failed,
+                      &gs->forced, &gs->thresh, &gs->order_thresh};
+    for (DevBuf* b : bufs) b->release();
+    delete gs;
+    return fail(-100 - (int)e, "grid state allocation failed: %s", hipGetErrorString(e));
+  }
+  g.grids.push_back(gs);
+  g.cur = gs;
   return 0;
 }
 
@@ -148,71 +205,208 @@ int order_blocks(int b0, int n, hipStream_t st) {
   HIP_TRY(g.sort_ids.ensure(sizeof(int) * n));
   HIP_TRY(g.order.ensure(sizeof(int) * n));
   hipLaunchKernelGGL(brams::block_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                     g.attempts.as<int>(), b0, n, g.sort_keys.as<int>(), g.sort_ids.as<int>());
+                     g.cur->attempts.as<int>(), b0, n, g.sort_keys.as<int>(), g.sort_ids.as<int>());
   HIP_TRY(hipGetLastError());
   return sort_desc(g.sort_keys.as<int>(), g.sort_ids.as<int>(), n, g.order.as<int>(), st);
 }
 
-// Launch the blocks of one grid; p has every pointer filled except the
-// per-call bookkeeping arrays, which this sets.
-int launch_step(brams::StepParams p, hipStream_t st, int nob) {
-  p.block_start = g.block_start.as<int>();
-  p.mask = g.mask.as<unsigned>();
-  p.blk_attempts = g.attempts.as<int>();
-  p.blk_substeps = g.substeps.as<int>();
-  p.blk_failed = g.failed.as<int>();
-  p.blk_forced = g.forced.as<float>();
-  const std::vector<int>& sz = g.part_sizes;
+// slots of a slab variant: every CU times the workgroups resident per CU
+int slab_slots(int variant, long long* nslots) {
+  const int per_cu = brams::slab_resident_per_cu(variant);
+  if (per_cu <= 0) return fail(-5, "slab kernel variant %d cannot be resident", variant);
+  *nslots = (long long)g.n_cu * per_cu;
+  return 0;
+}
+
+// per-slot scratch for n slots
+int ensure_slabs(long long n) {
+  if (n <= g.slab_slots) return 0;
+  HIP_TRY(g.slab_lu.ensure(sizeof(double) * 64 * brams::kSlabLuRows * n));
+  HIP_TRY(g.slab_jac.ensure(sizeof(double) * 64 * brams::kSlabJacRows * n));
+  HIP_TRY(g.slab_rk.ensure(sizeof(double) * 64 * brams::kSlabRkRows * n));
+  HIP_TRY(g.slab_vec.ensure(sizeof(double) * 64 * brams::kSlabVecRows * n));
+  g.slab_slots = n;
+  return 0;
+}
+
+brams::QueueParams queue(const int* order, int b0, int nitems) {
+  brams::QueueParams q{};
+  q.counter = g.qcount.as<int>();
+  q.order = order;
+  q.b0 = b0;
+  q.nitems = nitems;
+  q.lu = g.slab_lu.as<double>();
+  q.jac = g.slab_jac.as<double>();
+  q.rk = g.slab_rk.as<double>();
+  q.vec = g.slab_vec.as<double>();
+  return q;
+}
+
+// the 64-cell blocks [first64, nob) in the given kernel form; order: the
+// blocks' launch order (null: ascending)
+int launch_main(int form, const brams::StepParams& p, int first64, int nob, const int* order,
+                hipStream_t st) {
+  const int n = nob - first64;
+  if (n <= 0) return 0;
+  if (form == kFormPerLane) {
+    brams::StepParams pm = p;
+    pm.block_order = order;
+    HIP_TRY(brams::launch_rodas3(64, false, (unsigned)n, pm, first64, st));
+    return 0;
+  }
+  long long nslots = 0;
+  int rc = slab_slots(form, &nslots);
+  if (rc) return rc;
+  nslots = std::min<long long>(nslots, n);
+  if ((rc = ensure_slabs(nslots))) return rc;
+  HIP_TRY(brams::launch_rodas3_slab(form, (unsigned)nslots, p, queue(order, first64, n), st));
+  return 0;
+}
+
+// block 1 alone: its value-based mask fixes the mask of every later block
+int launch_block1(const brams::StepParams& p, hipStream_t st) {
+  if (!kSlabEnabled || g.cur->sizes[0] > 64) {
+    HIP_TRY(brams::launch_rodas3(g.cur->sizes[0] > 64 ? 128 : 64, true, 1, p, 0, st));
+    return 0;
+  }
+  if (g.cur->sizes[0] > 64) {
+    HIP_TRY(brams::launch_rodas3(128, true, 1, p, 0, st));
+    return 0;
+  }
+  int rc = ensure_slabs(1);
+  if (rc) return rc;
+  HIP_TRY(brams::launch_rodas3_slab(brams::kSlabBlock1, 1, p, queue(nullptr, 0, 1), st));
+  return 0;
+}
+
+// Autotuner: on the first call of a grid, each kernel form integrates the
+// call's 64-cell blocks on a scratch copy of the state (block 1 has run on
+// it first, so the mask is the call's), timed with events; the fastest is
+// kept for the grid.  The forms give bit-identical results, so the choice
+// changes speed only.  The caller's arrays are not touched.
+int autotune(brams::StepParams p, int first64, int nob, hipStream_t st) {
+  GridState& gs = *g.cur;
+  const size_t nsc = sizeof(float) * MECH_NS * (size_t)p.n3d;
+  const bool to_t = (p.split == brams::kSplitParallel) && (p.ndyn == 1);
+  HIP_TRY(g.tune_sc.ensure(2 * nsc * (to_t ? 2 : 1)));
+  HIP_TRY(g.tune_dt.ensure(2 * sizeof(double) * nob));
+  float* sc0 = g.tune_sc.as<float>();
+  float* sc1 = sc0 + (size_t)MECH_NS * p.n3d;
+  float* st0 = to_t ? sc1 + (size_t)MECH_NS * p.n3d : nullptr;
+  float* st1 = to_t ? st0 + (size_t)MECH_NS * p.n3d : nullptr;
+  double* dt0 = g.tune_dt.as<double>();
+  double* dt1 = dt0 + nob;
+  HIP_TRY(hipMemcpyAsync(sc0, p.sc_p, nsc, hipMemcpyDeviceToDevice, st));
+  if (to_t) HIP_TRY(hipMemcpyAsync(st0, p.sc_t, nsc, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(dt0, p.last_dt, sizeof(double) * nob, hipMemcpyDeviceToDevice, st));
+  brams::StepParams q = p;
+  q.sc_p = sc0;
+  q.sc_t = st0;
+  q.last_dt = dt0;
+  int rc = launch_block1(q, st);
+  if (rc) return rc;
+  q.sc_p = sc1;
+  q.sc_t = st1;
+  q.last_dt = dt1;
+  int best = 0;
+  for (int f = 0; f < kForms; ++f) {
+    HIP_TRY(hipMemcpyAsync(sc1, sc0, nsc, hipMemcpyDeviceToDevice, st));
+    if (to_t) HIP_TRY(hipMemcpyAsync(st1, st0, nsc, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dt1, dt0, sizeof(double) * nob, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipEventRecord(g.ev0, st));
+    if ((rc = launch_main(f, q, first64, nob, nullptr, st))) return rc;
+    HIP_TRY(hipEventRecord(g.ev1, st));
+    HIP_TRY(hipEventSynchronize(g.ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+    gs.tune_ms[f] = ms;
+    if (ms < gs.tune_ms[best]) best = f;
+  }
+  gs.form = best;
+  return 0;
+}
+
+// Launch the blocks of the current grid; p has every pointer filled except
+// the per-call bookkeeping arrays, which this sets.
+int launch_step(brams::StepParams p, hipStream_t st) {
+  GridState& gs = *g.cur;
+  const int nob = (int)gs.sizes.size();
+  p.block_start = gs.block_start.as<int>();
+  p.mask = gs.mask.as<unsigned>();
+  p.blk_attempts = gs.attempts.as<int>();
+  p.blk_substeps = gs.substeps.as<int>();
+  p.blk_failed = gs.failed.as<int>();
+  p.blk_forced = gs.forced.as<float>();
+  p.blk_thresh = gs.thresh.as<int>();
+  p.order_thresh = gs.order_thresh.as<int>();
+  const std::vector<int>& sz = gs.sizes;
   int nbig = 0;
   while (nbig < nob && sz[nbig] > 64) ++nbig;
   const int first64 = nbig > 1 ? nbig : 1;
-  // the 64-cell blocks run longest first once a call on this partition has
+  int form = g.force_form >= 0 ? g.force_form : gs.form;
+  if (!kSlabEnabled) form = kFormPerLane;
+  if (form < 0 && nob > first64) {
+    int rc = autotune(p, first64, nob, st);
+    if (rc) return rc;
+    form = gs.form;
+  }
+  // the 64-cell blocks run longest first once a call on this grid has
   // counted their attempts (the order changes no block's result)
-  brams::StepParams pm = p;
-  if (nob > first64 && g.blk_hist) {
+  const int* order = nullptr;
+  if (nob > first64 && gs.blk_hist) {
     int rc = order_blocks(first64, nob - first64, st);
     if (rc) return rc;
-    pm.block_order = g.order.as<int>();
+    order = g.order.as<int>();
   }
   HIP_TRY(hipEventRecord(g.ev0, st));
-  HIP_TRY(brams::launch_rodas3(sz[0] > 64 ? 128 : 64, true, 1, p, 0, st));
+  int rc = launch_block1(p, st);
+  if (rc) return rc;
   HIP_TRY(hipEventRecord(g.evb, st));
   if (nbig > 1) HIP_TRY(brams::launch_rodas3(128, false, (unsigned)(nbig - 1), p, 1, st));
   HIP_TRY(hipEventRecord(g.evm, st));
   g.main_cells = 0;
   if (nob > first64) {
-    HIP_TRY(brams::launch_rodas3(64, false, (unsigned)(nob - first64), pm, first64, st));
+    if ((rc = launch_main(form, p, first64, nob, order, st))) return rc;
     for (int b = first64; b < nob; ++b) g.main_cells += sz[b];
   }
   HIP_TRY(hipEventRecord(g.ev1, st));
-  g.blk_hist = true;
+  gs.blk_hist = true;
+  g.last_form = form;
   return 0;
 }
 
-int collect_stats(hipStream_t st, int nob, long long ncell) {
+int collect_stats(hipStream_t st) {
+  GridState& gs = *g.cur;
+  const int nob = (int)gs.sizes.size();
   HIP_TRY(hipStreamSynchronize(st));
-  g.h_att.resize(nob);
-  g.h_sub.resize(nob);
-  g.h_fail.resize(nob);
-  g.h_forced.resize(nob);
+  gs.h_att.resize(nob);
+  gs.h_sub.resize(nob);
+  gs.h_fail.resize(nob);
+  gs.h_forced.resize(nob);
+  gs.h_thresh.resize(nob);
   unsigned mw[MECH_MASK_WORDS];
-  HIP_TRY(hipMemcpy(g.h_att.data(), g.attempts.p, sizeof(int) * nob, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(g.h_sub.data(), g.substeps.p, sizeof(int) * nob, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(g.h_fail.data(), g.failed.p, sizeof(int) * nob, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(g.h_forced.data(), g.forced.p, sizeof(float) * nob, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(mw, g.mask.p, sizeof mw, hipMemcpyDeviceToHost));
+  int order_viol = 0;
+  HIP_TRY(hipMemcpy(gs.h_att.data(), gs.attempts.p, sizeof(int) * nob, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(gs.h_sub.data(), gs.substeps.p, sizeof(int) * nob, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(gs.h_fail.data(), gs.failed.p, sizeof(int) * nob, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(gs.h_forced.data(), gs.forced.p, sizeof(float) * nob, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(gs.h_thresh.data(), gs.thresh.p, sizeof(int) * nob, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(&order_viol, gs.order_thresh.p, sizeof(int), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(mw, gs.mask.p, sizeof mw, hipMemcpyDeviceToHost));
   brams_chem_stats s{};
-  s.cells = ncell;
+  s.cells = gs.ncell;
   s.blocks = nob;
   for (int b = 0; b < nob; ++b) {
-    s.attempts += g.h_att[b];
-    s.substeps += g.h_sub[b];
-    s.failed_blocks += g.h_fail[b];
-    s.cell_attempts += (long long)g.h_att[b] * g.part_sizes[b];
-    s.cell_substeps += (long long)g.h_sub[b] * g.part_sizes[b];
-    if (g.h_forced[b] > 1.0f) ++s.forced_blocks;
-    s.max_forced_err = std::max(s.max_forced_err, g.h_forced[b]);
+    s.attempts += gs.h_att[b];
+    s.substeps += gs.h_sub[b];
+    s.failed_blocks += gs.h_fail[b];
+    s.cell_attempts += (long long)gs.h_att[b] * gs.sizes[b];
+    s.cell_substeps += (long long)gs.h_sub[b] * gs.sizes[b];
+    if (gs.h_forced[b] > 1.0f) ++s.forced_blocks;
+    s.max_forced_err = std::max(s.max_forced_err, gs.h_forced[b]);
+    s.threshold_pivots += gs.h_thresh[b];
   }
+  s.order_threshold_pivots = order_viol;
   int nz = 0;
   for (int w = 0; w < MECH_MASK_WORDS; ++w) nz += __builtin_popcount(mw[w]);
   s.mask_nnz = nz;
@@ -224,6 +418,7 @@ int collect_stats(hipStream_t st, int nob, long long ncell) {
   HIP_TRY(hipEventElapsedTime(&ms, g.evm, g.ev1));
   s.main_ms = ms;
   s.main_cells = g.main_cells;
+  s.kernel_form = g.last_form;
   g.stats = s;
   return (int)s.failed_blocks;
 }
@@ -264,7 +459,10 @@ int brams_chem_init(int device) {
   if (!g.ev1) HIP_TRY(hipEventCreate(&g.ev1));
   if (!g.evb) HIP_TRY(hipEventCreate(&g.evb));
   if (!g.evm) HIP_TRY(hipEventCreate(&g.evm));
-  HIP_TRY(g.mask.ensure(sizeof(unsigned) * MECH_MASK_WORDS));
+  for (auto& e : g.sev)
+    if (!e) HIP_TRY(hipEventCreate(&e));
+  g.n_cu = prop.multiProcessorCount;
+  HIP_TRY(g.qcount.ensure(sizeof(int) * 64));
   g.init = true;
   return 0;
 }
@@ -303,9 +501,9 @@ int prepare_device(const brams_chem_dev_args* a, brams::StepParams& p, long long
     return fail(-2, "PARALLEL with N_DYN_CHEM=1 needs sc_t");
   if (a->split_method == BRAMS_SPLIT_PARALLEL && a->n_dyn_chem > 1 && !a->sc_t_dyn)
     return fail(-2, "PARALLEL with N_DYN_CHEM>1 needs sc_t_dyn");
-  ncell = (long long)(a->m1 - 2) * (a->iz - a->ia + 1) * (a->jz - a->ja + 1);
-  if ((rc = ensure_partition(ncell))) return rc;
-  const int nob = (int)g.part_sizes.size();
+  if ((rc = ensure_grid(a->m1, a->m2, a->m3, a->ia, a->iz, a->ja, a->jz))) return rc;
+  ncell = g.cur->ncell;
+  const int nob = (int)g.cur->sizes.size();
   if (a->nob != nob) return fail(-2, "nob %d does not match the AllocIndex partition (%d)", a->nob, nob);
   p = brams::StepParams{};
   p.m1 = a->m1; p.m2 = a->m2; p.m3 = a->m3;
@@ -332,10 +530,9 @@ int brams_chem_rodas3_dyndt_device(const brams_chem_dev_args* a) {
   long long ncell = 0;
   int rc = prepare_device(a, p, ncell);
   if (rc) return rc;
-  const int nob = (int)g.part_sizes.size();
   hipStream_t st = (hipStream_t)a->stream;
-  if ((rc = launch_step(p, st, nob))) return rc;
-  return collect_stats(st, nob, ncell);
+  if ((rc = launch_step(p, st))) return rc;
+  return collect_stats(st);
 }
 
 // ------------------------------------------------------------------ surrogate
@@ -396,11 +593,17 @@ int brams_sur_normalize(const float* raw, int64_t n, int ncol, const float* mu, 
 }
 
 int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams_sur_stats* out) {
+  if (!a) return fail(-2, "null args");
+  // the surrogate writes accepted cells straight into sc_p: the SYMMETRIC /
+  // SEQUENTIAL form of the write-back only (rodas3:774-798)
+  if (a->split_method == BRAMS_SPLIT_PARALLEL)
+    return fail(-2, "brams_sur_step supports SYMMETRIC and SEQUENTIAL split methods only");
+  if (!m || m->nmembers < 1 || m->nmembers > 4) return fail(-2, "surrogate needs 1..4 members");
   brams::StepParams p;
   long long ncell = 0;
   int rc = prepare_device(a, p, ncell);
   if (rc) return rc;
-  if (!m || m->nmembers < 1 || m->nmembers > 4) return fail(-2, "surrogate needs 1..4 members");
+  GridState& gs = *g.cur;
   hipStream_t st = (hipStream_t)a->stream;
   const long long nrows = (ncell + 63) / 64 * 64;
   const long long nchunk = (ncell + 1023) / 1024;
@@ -414,16 +617,15 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
   HIP_TRY(g.fb_dt.ensure(sizeof(double) * maxfb));
   HIP_TRY(g.fb_att.ensure(sizeof(int) * maxfb * 3));
   HIP_TRY(g.mask_all.ensure(sizeof(unsigned) * MECH_MASK_WORDS));
-  if (g.hist_cells != ncell) {
+  if (gs.hist_cells != ncell) {
     // fallback memory: each cell starts from its partition block's step
-    HIP_TRY(g.cell_dt.ensure(sizeof(double) * ncell));
+    HIP_TRY(gs.cell_dt.ensure(sizeof(double) * ncell));
     hipLaunchKernelGGL(brams::cell_history_init, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st,
-                       g.block_start.as<int>(), (int)a->nob, p.last_dt, ncell, g.cell_dt.as<double>());
+                       gs.block_start.as<int>(), (int)a->nob, p.last_dt, ncell, gs.cell_dt.as<double>());
     HIP_TRY(hipGetLastError());
-    g.hist_cells = ncell;
+    gs.hist_cells = ncell;
   }
-  hipEvent_t ev[5];
-  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  hipEvent_t* ev = g.sev;
   HIP_TRY(hipEventRecord(ev[0], st));
   // 1. features
   hipLaunchKernelGGL(brams::sur_features, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, st, p,
@@ -473,12 +675,12 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
   //    the cells by their previous cost instead was measured slower on the
   //    1.6M-cell rollout (68 against 57 ms): scattered gathers cost more than
   //    the more uniform blocks save.
-  long long fb_att = 0, fb_cell_att = 0, fb_cell_sub = 0;
+  long long fb_att = 0, fb_cell_att = 0, fb_cell_sub = 0, fb_failed = 0;
   if (nfb > 0) {
     const int nb = (nfb + 63) / 64;
     int* list = g.sur_list.as<int>();
     hipLaunchKernelGGL(brams::fallback_dt_hint, dim3((unsigned)nb), dim3(64), 0, st, list, nfb,
-                       g.cell_dt.as<double>(), (double)(a->dtlt * (float)a->n_dyn_chem),
+                       gs.cell_dt.as<double>(), (double)(a->dtlt * (float)a->n_dyn_chem),
                        g.fb_dt.as<double>());
     HIP_TRY(hipGetLastError());
     unsigned ones[MECH_MASK_WORDS];
@@ -493,20 +695,32 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
     q.blk_substeps = q.blk_attempts + maxfb;
     q.blk_failed = q.blk_attempts + 2 * maxfb;
     q.blk_forced = nullptr;
+    q.blk_thresh = nullptr;
+    q.order_thresh = nullptr;
     q.block_start = nullptr;
-    HIP_TRY(brams::launch_rodas3_list((unsigned)nb, q, st));
+    if (kSlabEnabled) {
+      long long nslots = 0;
+      if ((rc = slab_slots(brams::kSlabList, &nslots))) return rc;
+      nslots = std::min<long long>(nslots, nb);
+      if ((rc = ensure_slabs(nslots))) return rc;
+      HIP_TRY(brams::launch_rodas3_slab(brams::kSlabList, (unsigned)nslots, q, queue(nullptr, 0, nb), st));
+    } else {
+      HIP_TRY(brams::launch_rodas3_list((unsigned)nb, q, st));
+    }
     hipLaunchKernelGGL(brams::cell_history_update, dim3((unsigned)((nfb + 255) / 256)), dim3(256), 0, st, list,
-                       nfb, q.last_dt, g.cell_dt.as<double>());
+                       nfb, q.last_dt, gs.cell_dt.as<double>());
     HIP_TRY(hipGetLastError());
-    std::vector<int> att(nb), sub(nb);
+    std::vector<int> att(nb), sub(nb), bad(nb);
     HIP_TRY(hipMemcpyAsync(att.data(), q.blk_attempts, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(sub.data(), q.blk_substeps, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(bad.data(), q.blk_failed, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     for (int b = 0; b < nb; ++b) {
       const long long n = std::min(64, nfb - 64 * b);
       fb_att += att[b];
       fb_cell_att += n * att[b];
       fb_cell_sub += n * sub[b];
+      fb_failed += bad[b];
     }
   }
   HIP_TRY(hipEventRecord(ev[4], st));
@@ -519,6 +733,7 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
     s.fallback_block_attempts = fb_att;
     s.fallback_cell_attempts = fb_cell_att;
     s.fallback_cell_substeps = fb_cell_sub;
+    s.fallback_failed_blocks = fb_failed;
     float ms;
     HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); s.features_ms = ms;
     HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[2])); s.mlp_ms = ms;
@@ -527,11 +742,15 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
     HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[4])); s.total_ms = ms;
     *out = s;
   }
-  for (auto& e : ev) HIP_TRY(hipEventDestroy(e));
-  return 0;
+  // a fallback block that hit a zero pivot, a non-finite state or the
+  // attempt bound was left at its last accepted state: reported like the
+  // plain step's failed blocks
+  return (int)fb_failed;
 }
 
-void brams_sur_reset(void) { g.hist_cells = -1; }
+void brams_sur_reset(void) {
+  for (GridState* gs : g.grids) gs->hist_cells = -1;
+}
 
 int brams_chem_rodas3_dyndt(const brams_chem_args* a) {
   if (!a) return fail(-2, "null args");
@@ -545,14 +764,13 @@ int brams_chem_rodas3_dyndt(const brams_chem_args* a) {
   if (a->nspecies != MECH_NS || a->nr != MECH_NR || a->nr_photo != MECH_NPHOT)
     return fail(-2, "mechanism sizes %d/%d/%d differ from RELACS_TUV %d/%d/%d", a->nspecies, a->nr,
                 a->nr_photo, MECH_NS, MECH_NR, MECH_NPHOT);
-  const long long ncell = (long long)(a->m1 - 2) * (a->iz - a->ia + 1) * (a->jz - a->ja + 1);
-  if ((rc = ensure_partition(ncell))) return rc;
-  const int nob = (int)g.part_sizes.size();
+  if ((rc = ensure_grid(a->m1, a->m2, a->m3, a->ia, a->iz, a->ja, a->jz))) return rc;
+  const int nob = (int)g.cur->sizes.size();
   if (a->nob != nob) return fail(-2, "nob %d does not match the AllocIndex partition (%d)", a->nob, nob);
   for (int b = 0; b < nob; ++b)
-    if (a->block_end[b] != g.part_sizes[b])
+    if (a->block_end[b] != g.cur->sizes[b])
       return fail(-2, "block_end(%d)=%d differs from AllocIndex (%d)", b + 1, a->block_end[b],
-                  g.part_sizes[b]);
+                  g.cur->sizes[b]);
   // transport flags from the caller's index lists
   int flags[MECH_NS];
   for (int s = 0; s < MECH_NS; ++s) flags[s] = -1;
@@ -631,7 +849,7 @@ int brams_chem_rodas3_dyndt(const brams_chem_args* a) {
   p.sc_t_dyn = need_td ? g.sc_t_dyn.as<float>() : nullptr;
   p.n3d = n3d;
   p.last_dt = g.last_dt.as<double>();
-  if ((rc = launch_step(p, st, nob))) return rc;
+  if ((rc = launch_step(p, st))) return rc;
   // results back into the caller's arrays
   if (need_t) {
     for (int s = 0; s < MECH_NS; ++s)
@@ -642,19 +860,30 @@ int brams_chem_rodas3_dyndt(const brams_chem_args* a) {
   }
   HIP_TRY(hipMemcpyAsync(a->last_accepted_dt, g.last_dt.p, sizeof(double) * nob,
                          hipMemcpyDeviceToHost, st));
-  return collect_stats(st, nob, ncell);
+  return collect_stats(st);
 }
 
 int brams_chem_block_status(int32_t* attempts, int32_t* substeps, int32_t* failed,
                             float* forced_err, int32_t max_nob) {
-  const int nob = (int)g.h_att.size();
+  if (!g.cur) return fail(-2, "no chemistry step has run");
+  const GridState& gs = *g.cur;
+  const int nob = (int)gs.h_att.size();
   if (nob > max_nob) return fail(-2, "max_nob %d < nob %d", max_nob, nob);
   for (int b = 0; b < nob; ++b) {
-    if (attempts) attempts[b] = g.h_att[b];
-    if (substeps) substeps[b] = g.h_sub[b];
-    if (failed) failed[b] = g.h_fail[b];
-    if (forced_err) forced_err[b] = g.h_forced[b];
+    if (attempts) attempts[b] = gs.h_att[b];
+    if (substeps) substeps[b] = gs.h_sub[b];
+    if (failed) failed[b] = gs.h_fail[b];
+    if (forced_err) forced_err[b] = gs.h_forced[b];
   }
+  return nob;
+}
+
+int brams_chem_block_thresholds(int32_t* counts, int32_t max_nob) {
+  if (!g.cur) return fail(-2, "no chemistry step has run");
+  const GridState& gs = *g.cur;
+  const int nob = (int)gs.h_thresh.size();
+  if (nob > max_nob) return fail(-2, "max_nob %d < nob %d", max_nob, nob);
+  for (int b = 0; b < nob && counts; ++b) counts[b] = gs.h_thresh[b];
   return nob;
 }
 
@@ -667,21 +896,43 @@ int brams_chem_get_stats(brams_chem_stats* out) {
 const char* brams_chem_last_error(void) { return g_err.c_str(); }
 
 void brams_chem_finalize(void) {
-  DevBuf* bufs[] = {&g.block_start, &g.mask, &g.attempts, &g.substeps, &g.failed, &g.forced, &g.sc_p,
-                    &g.sc_t, &g.sc_t_dyn, &g.met, &g.jphoto, &g.last_dt, &g.consts,
+  for (GridState* gs : g.grids) {
+    DevBuf* gb[] = {&gs->block_start, &gs->mask, &gs->attempts, &gs->substeps, &gs->failed,
+                    &gs->forced, &gs->thresh, &gs->order_thresh, &gs->cell_dt};
+    for (DevBuf* b : gb) b->release();
+    delete gs;
+  }
+  g.grids.clear();
+  g.cur = nullptr;
+  DevBuf* bufs[] = {&g.sc_p, &g.sc_t, &g.sc_t_dyn, &g.met, &g.jphoto, &g.last_dt, &g.consts,
                     &g.sur_X, &g.sur_out, &g.sur_flag, &g.sur_list, &g.sur_chunk,
                     &g.sur_reason, &g.fb_dt, &g.fb_att, &g.mask_all, &g.sort_keys, &g.sort_ids,
-                    &g.sort_count, &g.sort_off, &g.order, &g.cell_dt};
+                    &g.sort_count, &g.sort_off, &g.order, &g.slab_lu, &g.slab_jac, &g.slab_rk,
+                    &g.slab_vec, &g.qcount, &g.tune_sc, &g.tune_dt};
   for (DevBuf* b : bufs) b->release();
-  hipEvent_t* evs[] = {&g.ev0, &g.ev1, &g.evb, &g.evm};
+  hipEvent_t* evs[] = {&g.ev0, &g.ev1, &g.evb, &g.evm, &g.sev[0], &g.sev[1], &g.sev[2], &g.sev[3],
+                       &g.sev[4]};
   for (hipEvent_t* e : evs) {
     if (*e) hipEventDestroy(*e);
     *e = nullptr;
   }
-  g.part_ncell = -1;
-  g.blk_hist = false;
-  g.hist_cells = -1;
+  g.slab_slots = 0;
   g.init = false;
+}
+
+int brams_chem_set_form(int32_t form) {
+  if (form < -1 || form >= kForms) return fail(-2, "kernel form %d out of range (-1..%d)", form, kForms - 1);
+  if (!kSlabEnabled && form >= 0 && form != kFormPerLane)
+    return fail(-6, "kernel form %d (slab) is switched off in this build", form);
+  g.force_form = form;
+  return 0;
+}
+
+int brams_chem_get_tuning(int32_t* form, float* ms, int32_t n) {
+  if (!g.cur) return fail(-2, "no chemistry step has run");
+  if (form) *form = g.cur->form;
+  for (int f = 0; f < n && f < kForms && ms; ++f) ms[f] = g.cur->tune_ms[f];
+  return kForms;
 }
 
 }  // extern "C"

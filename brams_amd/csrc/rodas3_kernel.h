@@ -42,6 +42,10 @@ struct StepParams {
   int* blk_substeps;             // [nob]
   int* blk_failed;               // [nob]
   float* blk_forced;             // [nob] largest error of an acceptance forced at dt_min (0: none)
+  int* blk_thresh;               // [nob] or null: pivots failing Sparse1.3's threshold rule, summed
+                                 // over the block's cells and attempts
+  int* order_thresh;             // [1] or null: the same count for cell 1 of block 1 (DYN kernel),
+                                 // the matrix the reference orders on (rodas3:470-506)
   const int* cell_list;          // optional: blocks are consecutive WG-sized runs of this list
   const int* block_order;        // optional: workgroup w integrates block block_order[w]
   int list_n;                    // length of cell_list
@@ -165,7 +169,7 @@ __global__ __launch_bounds__(WG, WPS) void rodas3_blocks(StepParams p, int block
   double dt = p.last_dt[b];
   double last_dt = dt;
   double time_c = 0.0;
-  int natt = 0, nsub = 0, failed = 0;
+  int natt = 0, nsub = 0, failed = 0, nviol = 0;
   float forced = 0.f;
 
   __shared__ unsigned s_mask[MECH_MASK_WORDS];
@@ -184,9 +188,11 @@ __global__ __launch_bounds__(WG, WPS) void rodas3_blocks(StepParams p, int block
       ++natt;
       const double igdt = 1.0 / (0.5 * dt);
       if constexpr (DYN) {
+        // every lane evaluates the mask of its own cell, so the evaluation
+        // is no divergent region; cell 1's is the one published
+        unsigned m[MECH_MASK_WORDS];
+        mech_mask(rk, y, igdt, m);
         if (c == 0) {
-          unsigned m[MECH_MASK_WORDS];
-          mech_mask(rk, y, igdt, m);
 #pragma unroll
           for (int w = 0; w < MECH_MASK_WORDS; ++w) s_mask[w] = m[w];
         }
@@ -195,7 +201,7 @@ __global__ __launch_bounds__(WG, WPS) void rodas3_blocks(StepParams p, int block
         for (int w = 0; w < MECH_MASK_WORDS; ++w) mw[w] = s_mask[w];
       }
       double lu[MECH_NLU];
-      const int bad = mech_factor(rk, y, igdt, mw, lu);
+      const int bad = mech_factor(rk, y, igdt, mw, lu, &nviol);
       const double dti = 1.0 / dt;
       // stage 1, rodas3:523-527
       double k1[MECH_NS];
@@ -305,6 +311,24 @@ __global__ __launch_bounds__(WG, WPS) void rodas3_blocks(StepParams p, int block
         p.sc_p[idx] = v > 0.0f ? v : 0.0f;
       }
     }
+  }
+  // threshold-rule count of the block: sum over its cells
+  {
+    int v = act ? nviol : 0;
+    if constexpr (DYN) {
+      if (c == 0 && p.order_thresh) p.order_thresh[0] = nviol;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if constexpr (WG > 64) {
+      __syncthreads();
+      if ((threadIdx.x & 63) == 0) s_rej[threadIdx.x >> 6] = v;
+      __syncthreads();
+      v = 0;
+#pragma unroll
+      for (int w = 0; w < WG / 64; ++w) v += s_rej[w];
+    }
+    if (c == 0 && p.blk_thresh) p.blk_thresh[b] = v;
   }
   if (c == 0) {
     p.last_dt[b] = last_dt;

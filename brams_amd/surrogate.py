@@ -67,7 +67,7 @@ class SurStats(C.Structure):
                 ("fallback_block_attempts", C.c_int64), ("fallback_cell_attempts", C.c_int64),
                 ("fallback_cell_substeps", C.c_int64), ("features_ms", C.c_float),
                 ("mlp_ms", C.c_float), ("apply_ms", C.c_float), ("fallback_ms", C.c_float),
-                ("total_ms", C.c_float)]
+                ("total_ms", C.c_float), ("fallback_failed_blocks", C.c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reasons"}
@@ -391,6 +391,8 @@ class SurrogateStep:
         _bind().brams_sur_reset()
 
     def step(self):
+        """One surrogate+fallback step; the stats carry the count of fallback
+        blocks that failed numerically (also brams_sur_step's return value)."""
         s = SurStats()
         abi.check(_bind().brams_sur_step(C.byref(self.ds.args), C.byref(self.cmodel), C.byref(s)),
                   "brams_sur_step")

@@ -25,7 +25,9 @@ Distributions (all drawn per cell unless said otherwise):
             RELACS relations J14 = 0.962055 J13 + 0.0106247 J9 and
             J16 = 12 J13 + 208 J15 (TUV/ModTuvDriver2.7.f90:1363-1368).
   SZA       "random": U(0,120) deg per column; "split": 30 -> 150 deg linear in
-            i (half day, half night).
+            i (half day, half night); "split_night_first": the same field
+            mirrored, 150 -> 30 deg, so that the reference's mask cell
+            (k=2, i=ia, j=ja, rodas3:470-482) is a night cell.
   plume     optional Gaussian patch (sigma in cells) centred in the domain in
             which NO, NO2 and the VOCs are multiplied by up to `plume_factor`.
 """
@@ -134,6 +136,8 @@ def make_workload(grid, seed=0, sza="random", plume_sigma=None, plume_factor=50.
         sza_col = rng.uniform(0.0, 120.0, (ni, nj))
     elif sza == "split":
         sza_col = np.broadcast_to((30.0 + 120.0 * np.arange(ni) / max(ni - 1, 1))[:, None], (ni, nj))
+    elif sza == "split_night_first":
+        sza_col = np.broadcast_to((150.0 - 120.0 * np.arange(ni) / max(ni - 1, 1))[:, None], (ni, nj))
     else:
         raise ValueError(sza)
     sza_col = sza_col + sza_offset

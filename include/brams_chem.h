@@ -111,6 +111,16 @@ typedef struct brams_chem_stats {
     int64_t main_cells;          /* cells that kernel integrated */
     int64_t forced_blocks;       /* blocks that accepted a step with error > 1 at dt_min */
     float max_forced_err;        /* largest such error (rodas3:671-676 accepts it) */
+    int64_t threshold_pivots;    /* pivots, over all cells and attempts, that Sparse1.3's
+                                    threshold rule would reject: |pivot| <= RelThreshold (1e-3)
+                                    x the largest other entry of its reduced column
+                                    (spFactor.c:202-203, spAllocate.c:191) */
+    int64_t order_threshold_pivots; /* the same count on cell 1 of block 1, the matrix the
+                                    reference orders on every block-1 attempt
+                                    (chem_spack_rodas3_dyndt.f90:470-506): 0 means Sparse's
+                                    threshold test accepts every pivot of the fixed order */
+    int32_t kernel_form;         /* kernel form that integrated the 64-cell blocks: 0..2 the
+                                    slab layouts, 3 the per-lane kernel */
 } brams_chem_stats;
 
 /* Select the HIP device and load the mechanism tables.  Idempotent. */
@@ -144,6 +154,23 @@ int brams_chem_get_stats(brams_chem_stats *out);
  * nob. */
 int brams_chem_block_status(int32_t *attempts, int32_t *substeps, int32_t *failed,
                             float *forced_err, int32_t max_nob);
+
+/* Per-block count of threshold-rule pivot failures of the last step (see
+ * brams_chem_stats.threshold_pivots); no reference counterpart.  Returns nob. */
+int brams_chem_block_thresholds(int32_t *counts, int32_t max_nob);
+
+/*
+ * Kernel form of the 64-cell blocks (no reference counterpart).  The forms
+ * give bit-identical results and differ in speed: 0..2 keep the per-cell
+ * working set in registers, LDS (32, 96 or 0 LU slots) and a per-slot
+ * scratch slab, run as persistent workgroups over a device work queue; 3 is
+ * the per-lane kernel.  -1 (default): the first call of each grid times every
+ * form on a scratch copy of its state and keeps the fastest for the grid.
+ */
+int brams_chem_set_form(int32_t form);
+/* The current grid's chosen form and the autotuner's timings (ms, one per
+ * form, 0 if not timed) into ms[0..n); returns the number of forms. */
+int brams_chem_get_tuning(int32_t *form, float *ms, int32_t n);
 
 /* Text of the last error (thread-local, never NULL). */
 const char *brams_chem_last_error(void);

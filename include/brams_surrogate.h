@@ -98,6 +98,8 @@ typedef struct brams_sur_stats {
     int64_t fallback_cell_attempts;  /* attempts x cells, summed over fallback blocks */
     int64_t fallback_cell_substeps;  /* accepted sub-steps x cells, likewise */
     float features_ms, mlp_ms, apply_ms, fallback_ms, total_ms;
+    int64_t fallback_failed_blocks;  /* fallback blocks ended by a zero pivot, a non-finite
+                                        state or the attempt bound (also the return value) */
 } brams_sur_stats;
 
 /* Raw features [ncell][BRAMS_SUR_NIN] f32 of the current state (dataset). */
@@ -113,7 +115,10 @@ int brams_sur_moments(const float *x, int64_t n, int ncol, double *sum, double *
 int brams_sur_normalize(const float *raw, int64_t n, int ncol, const float *mu, const float *sd,
                         void *out, int ldo, int out_bf16);
 /* Surrogate + fallback step: features, ensemble inference, checks, write-back
- * of accepted cells, FP64 RODAS3 on the rejected ones. */
+ * of accepted cells, FP64 RODAS3 on the rejected ones.  SYMMETRIC and
+ * SEQUENTIAL split methods only (PARALLEL is rejected with an error).
+ * Returns 0, the number of fallback blocks with a numerical failure (>0), or
+ * <0 on an error. */
 int brams_sur_step(const brams_chem_dev_args *a, const brams_sur_model *m, brams_sur_stats *st);
 /* Forget the fallback's per-cell memory of the last accepted step; the next
  * brams_sur_step starts it again from the partition blocks'

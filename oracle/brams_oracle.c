@@ -145,6 +145,12 @@ static unsigned char g_struct[NS][NS];   /* structural pattern incl. diagonal */
 static int g_perm[NS];                   /* internal -> species */
 static unsigned char g_fill[NS][NS];     /* internal pattern after fill-in */
 static int g_nnz_lu;
+/* the filled pattern as ascending index lists, so that factor and solve do
+ * the dense loops' work in the same order without scanning zeros */
+static int g_ucol_n[NS], g_ucol[NS][NS];   /* column J: K < J with fill[K][J] */
+static int g_lcol_n[NS], g_lcol[NS][NS];   /* column K: I > K with fill[I][K] */
+static int g_urow_n[NS], g_urow[NS][NS];   /* row I: J > I with fill[I][J] */
+static int g_nload, g_load_I[NS * NS], g_load_J[NS * NS], g_load_ij[NS * NS];
 
 static pthread_once_t g_once = PTHREAD_ONCE_INIT;
 
@@ -221,6 +227,22 @@ static void ora_init_tables(void)
             g_fill[I][J] = P[g_perm[I]][g_perm[J]];
             g_nnz_lu += g_fill[I][J];
         }
+    g_nload = 0;
+    for (int J = 0; J < NS; J++) {
+        g_ucol_n[J] = g_lcol_n[J] = g_urow_n[J] = 0;
+        for (int K = 0; K < J; K++)
+            if (g_fill[K][J]) g_ucol[J][g_ucol_n[J]++] = K;
+        for (int I = J + 1; I < NS; I++)
+            if (g_fill[I][J]) g_lcol[J][g_lcol_n[J]++] = I;
+        for (int K = J + 1; K < NS; K++)
+            if (g_fill[J][K]) g_urow[J][g_urow_n[J]++] = K;
+        for (int I = 0; I < NS; I++)
+            if (g_fill[I][J]) {
+                g_load_I[g_nload] = I;
+                g_load_J[g_nload] = J;
+                g_load_ij[g_nload++] = g_perm[I] * NS + g_perm[J];
+            }
+    }
 }
 
 static void ora_init(void) { pthread_once(&g_once, ora_init_tables); }
@@ -258,24 +280,38 @@ int ora_lu_order(int *perm, int *nnz_lu)
  * entries), backward with unit-diagonal U.  Works on internal indices. */
 typedef struct { double v[NS][NS]; } lumat;
 
-static int ora_lu_factor(lumat *m)
+/* Sparse1.3's threshold rule (spFactor.c:202-203 with FindBiggestInColExclude,
+ * spFactor.c:2019-2050): a pivot is kept only if RelThreshold (1e-3,
+ * spAllocate.c:191) times the largest other entry of its reduced column is
+ * below |pivot|.  Counts the pivots of the fixed order that fail it. */
+static int threshold_fails(const lumat *m, int J)
 {
-    double D[NS];
-    if (m->v[0][0] == 0.0) return 1;
-    m->v[0][0] = 1.0 / m->v[0][0];
-    for (int J = 1; J < NS; J++) {
-        for (int i = 0; i < NS; i++) D[i] = g_fill[i][J] ? m->v[i][J] : 0.0;
-        for (int K = 0; K < J; K++) {
-            if (!g_fill[K][J]) continue;
-            double u = D[K] * m->v[K][K];
+    if (g_lcol_n[J] == 0) return m->v[J][J] == 0.0;
+    double cm = fabs(m->v[g_lcol[J][0]][J]);
+    for (int t = 1; t < g_lcol_n[J]; t++) cm = fmax(cm, fabs(m->v[g_lcol[J][t]][J]));
+    return cm * 1.0e-3 >= fabs(m->v[J][J]);
+}
+
+/* Left-looking in place: column J receives, for K < J in its pattern in
+ * ascending order, U(K,J) = A(K,J) / pivot_K (as a product with the stored
+ * reciprocal) and A(I,J) -= U(K,J) L(I,K) for I > K in ascending order;
+ * then its pivot is replaced by the reciprocal.  Entries outside the filled
+ * pattern are never read. */
+static int ora_lu_factor(lumat *m, int *viol)
+{
+    for (int J = 0; J < NS; J++) {
+        for (int a = 0; a < g_ucol_n[J]; a++) {
+            const int K = g_ucol[J][a];
+            const double u = m->v[K][J] * m->v[K][K];
             m->v[K][J] = u;
-            for (int I = K + 1; I < NS; I++)
-                if (g_fill[I][K]) D[I] -= u * m->v[I][K];
+            for (int b = 0; b < g_lcol_n[K]; b++) {
+                const int I = g_lcol[K][b];
+                m->v[I][J] -= u * m->v[I][K];
+            }
         }
-        for (int I = J + 1; I < NS; I++)
-            if (g_fill[I][J]) m->v[I][J] = D[I];
-        if (D[J] == 0.0) return J + 1;
-        m->v[J][J] = 1.0 / D[J];
+        *viol += threshold_fails(m, J);
+        if (m->v[J][J] == 0.0) return J + 1;
+        m->v[J][J] = 1.0 / m->v[J][J];
     }
     return 0;
 }
@@ -289,38 +325,63 @@ static void ora_lu_solve(const lumat *m, const double *rhs, double *x)
         if (t != 0.0) {
             t *= m->v[I][I];
             c[I] = t;
-            for (int R = I + 1; R < NS; R++)
-                if (g_fill[R][I]) c[R] -= t * m->v[R][I];
+            for (int b = 0; b < g_lcol_n[I]; b++) {
+                const int R = g_lcol[I][b];
+                c[R] -= t * m->v[R][I];
+            }
         }
     }
     for (int I = NS - 1; I >= 0; I--) {
         double t = c[I];
-        for (int J = I + 1; J < NS; J++)
-            if (g_fill[I][J]) t -= m->v[I][J] * c[J];
+        for (int b = 0; b < g_urow_n[I]; b++) {
+            const int J = g_urow[I][b];
+            t -= m->v[I][J] * c[J];
+        }
         c[I] = t;
     }
     for (int I = 0; I < NS; I++) x[g_perm[I]] = c[I];
 }
 
-/* load a masked species-indexed matrix into internal order */
+/* load a masked species-indexed matrix into internal order (filled pattern) */
 static void ora_lu_load(lumat *m, const double *a, const unsigned char *mask)
 {
-    for (int I = 0; I < NS; I++)
-        for (int J = 0; J < NS; J++) {
-            int i = g_perm[I], j = g_perm[J];
-            m->v[I][J] = (g_fill[I][J] && mask[i * NS + j]) ? a[i * NS + j] : 0.0;
-        }
+    for (int t = 0; t < g_nload; t++) {
+        const int ij = g_load_ij[t];
+        m->v[g_load_I[t]][g_load_J[t]] = mask[ij] ? a[ij] : 0.0;
+    }
+}
+
+/* the matrix 1/(gamma h) I - J of one cell, masked, straight into internal order */
+static void ora_lu_load_jac(lumat *m, const double *jac, double igdt, const unsigned char *mask)
+{
+    for (int t = 0; t < g_nload; t++) {
+        const int ij = g_load_ij[t];
+        const int i = ij / NS, j = ij % NS;
+        const double v = (i == j) ? igdt - jac[ij] : -jac[ij];
+        m->v[g_load_I[t]][g_load_J[t]] = mask[ij] ? v : 0.0;
+    }
 }
 
 int ora_lu_solve_dense(const double *a, const unsigned char *mask, const double *b, double *x)
 {
     ora_init();
     lumat m;
+    int viol = 0;
     ora_lu_load(&m, a, mask);
-    int rc = ora_lu_factor(&m);
+    int rc = ora_lu_factor(&m, &viol);
     if (rc) return rc;
     ora_lu_solve(&m, b, x);
     return 0;
+}
+
+int ora_lu_threshold_fails(const double *a, const unsigned char *mask)
+{
+    ora_init();
+    lumat m;
+    int viol = 0;
+    ora_lu_load(&m, a, mask);
+    ora_lu_factor(&m, &viol);
+    return viol;
 }
 
 /* external Sparse1.3 hooks */
@@ -363,6 +424,10 @@ typedef struct {
     int64_t *attempts, *substeps;
     int *failed;
     float *forced;
+    int *thresh;                 /* per block: threshold-rule pivot failures */
+    int order_viol;              /* the same on cell 1 of block 1 */
+    const int32_t *list;         /* blocks to run after block 1 (NULL: all) */
+    int nlist;
     atomic_int next;
     int has_sparse;              /* external Sparse1.3 in use */
     long sp_handle;
@@ -384,6 +449,8 @@ typedef struct {
 
 static int64_t *g_last_attempts, *g_last_substeps;
 static float *g_last_forced;
+static int *g_last_thresh;
+static int *g_last_failed;
 static int g_last_nob;
 
 static void cell_offset(const brams_chem_args *a, long g, long *off)
@@ -629,6 +696,40 @@ int ora_rodas3_linear(int n, const double *A, const double *y0, double h, int ns
     return 0;
 }
 
+/* Test hooks of the controller (block 1 only, single-threaded use):
+ * g_script replaces the block's largest error of attempt i by
+ * g_script[min(i, n-1)] (every cell gets that error, so the accept test sees
+ * it too); g_trace records each attempt's (dt, accepted, time after). */
+static const double *g_script;
+static int g_nscript;
+#define ORA_TRACE_MAX 4096
+static double g_trace_dt[ORA_TRACE_MAX], g_trace_t[ORA_TRACE_MAX];
+static int g_trace_acc[ORA_TRACE_MAX], g_ntrace;
+static atomic_long g_jac_calls, g_f0_calls;
+
+void ora_set_error_script(const double *errs, int n)
+{
+    g_script = errs;
+    g_nscript = (errs && n > 0) ? n : 0;
+}
+
+int ora_controller_trace(double *dt, int *accepted, double *time_after, int max)
+{
+    int n = g_ntrace < max ? g_ntrace : max;
+    for (int i = 0; i < n; i++) {
+        dt[i] = g_trace_dt[i];
+        accepted[i] = g_trace_acc[i];
+        time_after[i] = g_trace_t[i];
+    }
+    return g_ntrace;
+}
+
+void ora_call_counts(int64_t *jacobian_calls, int64_t *f0_calls)
+{
+    *jacobian_calls = atomic_load(&g_jac_calls);
+    *f0_calls = atomic_load(&g_f0_calls);
+}
+
 /* RODAS3 on one block, chem_spack_rodas3_dyndt.f90:413-705 */
 static int integrate_block(drv_t *D, blk_t *B, int b)
 {
@@ -639,9 +740,10 @@ static int integrate_block(drv_t *D, blk_t *B, int b)
     const double dt_max = (double)tn;
     double time_c = 0.0;
     const double time_f = (double)tn;
-    int failed = 0;
+    int failed = 0, viol = 0;
     int64_t natt = 0, nsub = 0;
     float forced = 0.f;
+    if (b == 0) g_ntrace = 0;
 
     for (int c = 0; c < B->n; c++)
         ora_kinetic(B->temp[c], B->press[c], B->vapp[c], B->jph[c], B->rk[c]);
@@ -651,6 +753,8 @@ static int integrate_block(drv_t *D, blk_t *B, int b)
             ora_jacobian(B->rk[c], B->y[c], B->jac[c]);
             ora_fexchem(B->rk[c], B->y[c], B->f0[c]);
         }
+        atomic_fetch_add(&g_jac_calls, B->n);
+        atomic_fetch_add(&g_f0_calls, B->n);
         for (;;) {
             natt++;
             const double igdt = 1.0 / (0.5 * dt);
@@ -674,18 +778,21 @@ static int integrate_block(drv_t *D, blk_t *B, int b)
                 }
             }
             for (int c = 0; c < B->n; c++) {
-                double mat[NS * NS];
-                for (int i = 0; i < NS; i++)
-                    for (int j = 0; j < NS; j++)
-                        mat[i * NS + j] = (i == j) ? igdt - B->jac[c][i * NS + j] : -B->jac[c][i * NS + j];
                 if (D->has_sparse) {
+                    double mat[NS * NS];
+                    for (int i = 0; i < NS; i++)
+                        for (int j = 0; j < NS; j++)
+                            mat[i * NS + j] = (i == j) ? igdt - B->jac[c][i * NS + j] : -B->jac[c][i * NS + j];
                     double vals[NS * NS];
                     for (int z = 0; z < D->sp_nnz; z++)
                         vals[z] = mat[(D->sp_ipos[z] - 1) * NS + D->sp_jpos[z] - 1];
                     if (g_sp_load_factor(D->sp_handle, D->sp_nnz, D->sp_elem, vals) != 0) failed = 1;
                 } else {
-                    ora_lu_load(&B->lu, mat, D->mask);
-                    if (ora_lu_factor(&B->lu) != 0) failed = 1;
+                    int v = 0;
+                    ora_lu_load_jac(&B->lu, B->jac[c], igdt, D->mask);
+                    if (ora_lu_factor(&B->lu, &v) != 0) failed = 1;
+                    viol += v;
+                    if (b == 0 && c == 0) D->order_viol += v;
                 }
                 stage_ctx_t ctx = {D, &B->lu, B->rk[c]};
                 rodas3_stages(NS, &ctx, stage_fun, stage_solve, B->y[c], B->f0[c], dt,
@@ -698,6 +805,8 @@ static int integrate_block(drv_t *D, blk_t *B, int b)
                 double e = rodas3_solution(NS, B->y[c], B->k1[c], B->k3[c], B->k4[c], a->atol, a->rtol,
                                            B->ynew[c], &B->err[c]);
                 if (!isfinite(e)) failed = 1;
+                if (b == 0 && g_nscript > 0)
+                    B->err[c] = g_script[(natt - 1) < g_nscript ? (natt - 1) : g_nscript - 1];
                 if (B->err[c] - 1.0e-8 > 1.0) all_accepted = 0;
                 if (c == 0 || B->err[c] > max_err) max_err = B->err[c];
             }
@@ -705,11 +814,18 @@ static int integrate_block(drv_t *D, blk_t *B, int b)
             double dt_new = rodas3_dt_new(dt, max_err, dt_min, dt_max);
             if (failed) { time_c = time_f; break; }
             if (!all_accepted && dt_new > dt_min) {
+                if (b == 0 && g_ntrace < ORA_TRACE_MAX) {
+                    g_trace_dt[g_ntrace] = dt; g_trace_acc[g_ntrace] = 0; g_trace_t[g_ntrace++] = time_c;
+                }
                 dt = dt_new;
                 continue;
             }
             /* error above 1 accepted at dt_min, rodas3:671-676 */
             if (!all_accepted && max_err > forced) forced = (float)max_err;
+            if (b == 0 && g_ntrace < ORA_TRACE_MAX) {
+                g_trace_dt[g_ntrace] = dt; g_trace_acc[g_ntrace] = all_accepted ? 1 : 2;
+                g_trace_t[g_ntrace++] = time_c + dt;
+            }
             time_c = time_c + dt;
             dt = fmin(dt_new, time_f - time_c);
             if (time_c < time_f) a->last_accepted_dt[b] = dt_new;
@@ -722,6 +838,7 @@ static int integrate_block(drv_t *D, blk_t *B, int b)
     D->substeps[b] = nsub;
     D->failed[b] = failed;
     D->forced[b] = forced;
+    D->thresh[b] = viol;
     return failed;
 }
 
@@ -739,16 +856,24 @@ static void *worker(void *arg)
     drv_t *D = (drv_t *)arg;
     blk_t *B = (blk_t *)malloc(sizeof(blk_t));
     for (;;) {
-        int b = atomic_fetch_add(&D->next, 1);
-        if (b >= D->nob) break;
+        int k = atomic_fetch_add(&D->next, 1);
+        int b;
+        if (D->list) {
+            if (k >= D->nlist) break;
+            b = D->list[k];
+            if (b <= 0 || b >= D->nob) continue;   /* block 1 ran first; ignore bad ids */
+        } else {
+            if (k >= D->nob) break;
+            b = k;
+        }
         run_block(D, B, b);
     }
     free(B);
     return NULL;
 }
 
-int ora_rodas3_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
-                     unsigned char *mask_out)
+static int run_step(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
+                    unsigned char *mask_out, const int32_t *list, int nlist)
 {
     ora_init();
     if (a->nspecies != NS || a->nr != NR || a->nr_photo != NP) return -1;
@@ -757,17 +882,22 @@ int ora_rodas3_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *s
     drv_t *D = (drv_t *)calloc(1, sizeof(drv_t));
     D->a = a;
     D->nob = a->nob;
+    D->list = list;
+    D->nlist = nlist;
     D->bstart = (long *)malloc(sizeof(long) * (a->nob + 1));
     D->attempts = (int64_t *)calloc(a->nob, sizeof(int64_t));
     D->substeps = (int64_t *)calloc(a->nob, sizeof(int64_t));
     D->failed = (int *)calloc(a->nob, sizeof(int));
     D->forced = (float *)calloc(a->nob, sizeof(float));
+    D->thresh = (int *)calloc(a->nob, sizeof(int));
     D->bstart[0] = 0;
     for (int b = 0; b < a->nob; b++) D->bstart[b + 1] = D->bstart[b] + a->block_end[b];
     D->has_sparse = (g_sp_create && g_sp_load_factor && g_sp_solve);
     D->sp_elem = (long *)malloc(sizeof(long) * NS * NS);
     D->sp_ipos = (int *)malloc(sizeof(int) * NS * NS);
     D->sp_jpos = (int *)malloc(sizeof(int) * NS * NS);
+    atomic_store(&g_jac_calls, 0);
+    atomic_store(&g_f0_calls, 0);
 
     /* block 1 first: it fixes the mask (and, in the reference, the order) */
     blk_t *B = (blk_t *)malloc(sizeof(blk_t));
@@ -775,7 +905,7 @@ int ora_rodas3_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *s
     free(B);
     if (mask_out) memcpy(mask_out, D->mask, NS * NS);
 
-    atomic_store(&D->next, 1);
+    atomic_store(&D->next, list ? 0 : 1);
     if (nthreads <= 0) nthreads = (int)sysconf(_SC_NPROCESSORS_ONLN);
     if (D->has_sparse) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
@@ -784,42 +914,74 @@ int ora_rodas3_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *s
     for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
 
     int nfail = 0;
-    int64_t att = 0, sub = 0, cells = 0;
+    int64_t att = 0, sub = 0, cells = 0, viol = 0;
+    char *ran = (char *)calloc(a->nob, 1);
+    ran[0] = 1;
+    if (list) {
+        for (int k = 0; k < nlist; k++)
+            if (list[k] > 0 && list[k] < a->nob) ran[list[k]] = 1;
+    } else {
+        memset(ran, 1, a->nob);
+    }
     for (int b = 0; b < a->nob; b++) {
+        if (!ran[b]) continue;
         nfail += D->failed[b];
         att += D->attempts[b];
         sub += D->substeps[b];
         cells += a->block_end[b];
+        viol += D->thresh[b];
     }
     if (st) {
         memset(st, 0, sizeof *st);
         st->cells = cells;
-        st->blocks = a->nob;
+        for (int b = 0; b < a->nob; b++) st->blocks += ran[b];
         st->substeps = sub;
         st->attempts = att;
         st->failed_blocks = nfail;
         for (int b = 0; b < a->nob; b++) {
+            if (!ran[b]) continue;
             if (D->forced[b] > 1.0f) st->forced_blocks++;
             if (D->forced[b] > st->max_forced_err) st->max_forced_err = D->forced[b];
+            st->cell_attempts += D->attempts[b] * a->block_end[b];
+            st->cell_substeps += D->substeps[b] * a->block_end[b];
         }
         int nz = 0;
         for (int i = 0; i < NS * NS; i++) nz += D->mask[i];
         st->mask_nnz = nz;
+        st->threshold_pivots = viol;
+        st->order_threshold_pivots = D->order_viol;
     }
+    free(ran);
     free(g_last_attempts);
     free(g_last_substeps);
     free(g_last_forced);
+    free(g_last_thresh);
+    free(g_last_failed);
     g_last_attempts = D->attempts;
     g_last_substeps = D->substeps;
     g_last_forced = D->forced;
+    g_last_thresh = D->thresh;
+    g_last_failed = D->failed;
     g_last_nob = a->nob;
-    free(D->failed);
     free(D->bstart);
     free(D->sp_elem);
     free(D->sp_ipos);
     free(D->sp_jpos);
     free(D);
     return nfail;
+}
+
+int ora_rodas3_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
+                     unsigned char *mask_out)
+{
+    return run_step(a, nthreads, st, mask_out, NULL, 0);
+}
+
+int ora_rodas3_dyndt_blocks(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
+                            unsigned char *mask_out, const int32_t *blocks, int nblocks)
+{
+    static const int32_t none[1] = {0};
+    return run_step(a, nthreads, st, mask_out, blocks ? blocks : none, blocks ? nblocks : 0);
 }
 
 int ora_block_counts(int64_t *attempts, int64_t *substeps, int max_nob)
@@ -837,4 +999,50 @@ int ora_block_forced(float *forced, int max_nob)
     int n = g_last_nob < max_nob ? g_last_nob : max_nob;
     for (int b = 0; b < n; b++) forced[b] = g_last_forced[b];
     return g_last_nob;
+}
+
+int ora_block_thresholds(int32_t *counts, int max_nob)
+{
+    int n = g_last_nob < max_nob ? g_last_nob : max_nob;
+    for (int b = 0; b < n; b++) counts[b] = g_last_thresh[b];
+    return g_last_nob;
+}
+
+int ora_block_failed(int32_t *failed, int max_nob)
+{
+    int n = g_last_nob < max_nob ? g_last_nob : max_nob;
+    for (int b = 0; b < n; b++) failed[b] = g_last_failed[b];
+    return g_last_nob;
+}
+
+/* One cell's gather (rodas3:226-354) and write-back (rodas3:713-798) alone,
+ * for unit tests of the conversions: cell is the global interior index in
+ * j -> i -> k order. */
+int ora_gather_cell(const brams_chem_args *a, long cell, double *y, double *temp, double *press,
+                    double *volmol)
+{
+    blk_t *B = (blk_t *)malloc(sizeof(blk_t));
+    if (!B) return -1;
+    B->n = 1;
+    cell_offset(a, cell, &B->off[0]);
+    gather(a, B);
+    memcpy(y, B->y[0], sizeof(double) * NS);
+    *temp = B->temp[0];
+    *press = B->press[0];
+    *volmol = B->volmol[0];
+    free(B);
+    return 0;
+}
+
+int ora_scatter_cell(const brams_chem_args *a, long cell, const double *y)
+{
+    blk_t *B = (blk_t *)malloc(sizeof(blk_t));
+    if (!B) return -1;
+    B->n = 1;
+    cell_offset(a, cell, &B->off[0]);
+    gather(a, B);                 /* volmol of the cell */
+    memcpy(B->y[0], y, sizeof(double) * NS);
+    scatter(a, B);
+    free(B);
+    return 0;
 }

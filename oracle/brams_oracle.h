@@ -38,6 +38,9 @@ void ora_jacobian(const double *rk, const double *y, double *jac);
 int ora_lu_order(int *perm, int *nnz_lu);
 /* Factor the masked matrix a (row-major 47x47, mask 0/1) and solve a x = b. */
 int ora_lu_solve_dense(const double *a, const unsigned char *mask, const double *b, double *x);
+/* Pivots of the fixed order that fail Sparse1.3's threshold rule when the
+ * masked matrix a is factored (the count the device kernels report). */
+int ora_lu_threshold_fails(const double *a, const unsigned char *mask);
 
 /* Block partition, AllocIndex (chem_spack_utils.f90:39-183); returns nob. */
 int ora_alloc_index(int block_size, int m1, int ia, int iz, int ja, int jz,
@@ -47,6 +50,12 @@ int ora_alloc_index(int block_size, int m1, int ia, int iz, int ja, int jz,
  * NULL) receives the value-based sparsity mask blocks 2..nob used. */
 int ora_rodas3_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
                      unsigned char *mask_out);
+
+/* The same step on block 1 and the listed blocks only (0-based ids; others
+ * are left untouched).  Blocks are independent once block 1 has fixed the
+ * mask, so each listed block's result equals the full step's. */
+int ora_rodas3_dyndt_blocks(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
+                            unsigned char *mask_out, const int32_t *blocks, int nblocks);
 
 /* The RODAS3 stages, solution and error norm of the step applied to the
  * linear problem y' = A y (dense n x n, n <= 47) with nsteps fixed steps h;
@@ -60,6 +69,26 @@ double ora_dt_new(double dt, double max_err, double dt_min, double dt_max);
 int ora_block_counts(int64_t *attempts, int64_t *substeps, int max_nob);
 /* Per-block largest error accepted at dt_min (0: none) of the last step. */
 int ora_block_forced(float *forced, int max_nob);
+/* Per-block count of pivots failing Sparse1.3's threshold rule, last step. */
+int ora_block_thresholds(int32_t *counts, int max_nob);
+/* per-block failure flags of the last step (a zero pivot or a non-finite
+ * error sum: the point where the reference stops the model) */
+int ora_block_failed(int32_t *failed, int max_nob);
+
+/* Test hooks.  One cell's gather (Exner -> T, p; ppbm -> molec/cm^3) and
+ * write-back alone; cell is the interior index in j -> i -> k order. */
+int ora_gather_cell(const brams_chem_args *a, long cell, double *y, double *temp, double *press,
+                    double *volmol);
+int ora_scatter_cell(const brams_chem_args *a, long cell, const double *y);
+/* Replace block 1's largest error of attempt i by errs[min(i, n-1)]
+ * (NULL/0 switches it off); run single-threaded with a one-block grid. */
+void ora_set_error_script(const double *errs, int n);
+/* Block 1's attempts of the last step: dt tried, accepted (0 rejected, 1
+ * accepted, 2 accepted with error > 1 at dt_min), time after the attempt.
+ * Returns the number of attempts. */
+int ora_controller_trace(double *dt, int *accepted, double *time_after, int max);
+/* Jacobian and f(y0) evaluations (cells x sub-steps) of the last step. */
+void ora_call_counts(int64_t *jacobian_calls, int64_t *f0_calls);
 
 /* Route the LU through an external Sparse1.3 build (the reference's, via
  * oracle/ref_sparse_harness.c).  NULL pointers restore the oracle's own LU.

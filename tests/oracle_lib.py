@@ -51,7 +51,9 @@ class ChemStats(C.Structure):
                 ("mask_nnz", C.c_int32), ("kernel_ms", C.c_float),
                 ("cell_substeps", C.c_int64), ("cell_attempts", C.c_int64),
                 ("block1_ms", C.c_float), ("main_ms", C.c_float), ("main_cells", C.c_int64),
-                ("forced_blocks", C.c_int64), ("max_forced_err", C.c_float)]
+                ("forced_blocks", C.c_int64), ("max_forced_err", C.c_float),
+                ("threshold_pivots", C.c_int64), ("order_threshold_pivots", C.c_int64),
+                ("kernel_form", C.c_int32)]
 
 
 _oracle = None
@@ -268,6 +270,92 @@ def run_oracle(wl, atol, rtol, nthreads=0, split_method=0, n_dyn_chem=1, sc_t=No
         if reference_sparse:
             use_reference_sparse(False)
     return hs, st, mask.reshape(NS, NS), rc
+
+
+def run_oracle_blocks(wl, atol, rtol, blocks, nthreads=0):
+    """The oracle step on block 1 plus the listed blocks (0-based); the other
+    blocks' cells are left as they were.  Returns (HostStep, stats, mask, rc)."""
+    hs = HostStep(wl, atol, rtol)
+    st = ChemStats()
+    mask = np.zeros(NS * NS, np.uint8)
+    lib = oracle()
+    lib.ora_rodas3_dyndt_blocks.argtypes = [C.POINTER(ChemArgs), C.c_int, C.POINTER(ChemStats),
+                                            C.POINTER(C.c_uint8), _ip, C.c_int]
+    ids = np.ascontiguousarray(blocks, dtype=np.int32)
+    rc = lib.ora_rodas3_dyndt_blocks(C.byref(hs.args), nthreads, C.byref(st),
+                                     ptr(mask, C.POINTER(C.c_uint8)), ptr(ids, _ip), len(ids))
+    return hs, st, mask.reshape(NS, NS), rc
+
+
+def block_thresholds(nob):
+    c = np.zeros(nob, np.int32)
+    lib = oracle()
+    lib.ora_block_thresholds.argtypes = [_ip, C.c_int]
+    lib.ora_block_thresholds(ptr(c, _ip), nob)
+    return c
+
+
+def block_failed(nob):
+    """Per-block failure flags of the last oracle step."""
+    c = np.zeros(nob, np.int32)
+    lib = oracle()
+    lib.ora_block_failed.argtypes = [_ip, C.c_int]
+    lib.ora_block_failed(ptr(c, _ip), nob)
+    return c
+
+
+def gather_cell(wl, cell, atol=None, rtol=None):
+    """One cell's gathered state: (y[47] molec/cm^3, T, p, volmol)."""
+    hs = HostStep(wl, np.ones(NS) if atol is None else atol, np.ones(NS) if rtol is None else rtol)
+    lib = oracle()
+    lib.ora_gather_cell.argtypes = [C.POINTER(ChemArgs), C.c_long, _dp, _dp, _dp, _dp]
+    y = np.zeros(NS); t = np.zeros(1); p = np.zeros(1); v = np.zeros(1)
+    assert lib.ora_gather_cell(C.byref(hs.args), cell, ptr(y), ptr(t), ptr(p), ptr(v)) == 0
+    return y, float(t[0]), float(p[0]), float(v[0])
+
+
+def scatter_cell(wl, cell, y):
+    """Write one cell's state y[47] (molec/cm^3) back; returns the species
+    arrays after the write-back."""
+    hs = HostStep(wl, np.ones(NS), np.ones(NS))
+    lib = oracle()
+    lib.ora_scatter_cell.argtypes = [C.POINTER(ChemArgs), C.c_long, _dp]
+    yy = np.ascontiguousarray(y, dtype=np.float64)
+    assert lib.ora_scatter_cell(C.byref(hs.args), cell, ptr(yy)) == 0
+    return hs.sc_p
+
+
+def set_error_script(errs):
+    """Scripted block-1 errors for the controller hooks (None switches off)."""
+    global _script_keep
+    lib = oracle()
+    lib.ora_set_error_script.argtypes = [_dp, C.c_int]
+    if errs is None:
+        _script_keep = None
+        lib.ora_set_error_script(None, 0)
+    else:
+        _script_keep = np.ascontiguousarray(errs, dtype=np.float64)
+        lib.ora_set_error_script(ptr(_script_keep), len(_script_keep))
+
+
+_script_keep = None
+
+
+def controller_trace(max_n=4096):
+    lib = oracle()
+    lib.ora_controller_trace.argtypes = [_dp, _ip, _dp, C.c_int]
+    dt = np.zeros(max_n); acc = np.zeros(max_n, np.int32); t = np.zeros(max_n)
+    n = lib.ora_controller_trace(ptr(dt), ptr(acc, _ip), ptr(t), max_n)
+    n = min(n, max_n)
+    return dt[:n], acc[:n], t[:n]
+
+
+def call_counts():
+    lib = oracle()
+    lib.ora_call_counts.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    j = C.c_int64(); f = C.c_int64()
+    lib.ora_call_counts(C.byref(j), C.byref(f))
+    return j.value, f.value
 
 
 def block_forced(nob):

@@ -52,3 +52,33 @@ def test_invalid_arguments_are_reported_not_aborted():
     rc = L.brams_chem_alloc_index(64, 2, 2, 1, 2, 17, None, 0)
     assert rc < 0                      # m1=2 -> no interior levels
     assert b"empty" in L.brams_chem_last_error()
+
+
+def test_surrogate_step_rejects_parallel_split():
+    """brams_sur_step writes accepted cells straight into sc_p (the
+    SYMMETRIC/SEQUENTIAL write-back, rodas3:774-798), so a PARALLEL call is
+    refused with an error before any device work."""
+    from brams_amd.surrogate import SurModel, SurStats
+    L = abi.lib()
+    L.brams_sur_step.argtypes = [C.POINTER(abi.ChemDevArgs), C.POINTER(SurModel), C.POINTER(SurStats)]
+    a = abi.ChemDevArgs()
+    a.m1, a.m2, a.m3, a.ia, a.iz, a.ja, a.jz = 12, 18, 18, 2, 17, 2, 17
+    a.split_method = abi.SPLIT["PARALLEL"]
+    a.n_dyn_chem = 1
+    m = SurModel()
+    m.nmembers = 3
+    rc = L.brams_sur_step(C.byref(a), C.byref(m), None)
+    assert rc < 0
+    assert b"SYMMETRIC and SEQUENTIAL" in L.brams_chem_last_error()
+
+
+def test_slab_forms_are_refused_while_switched_off():
+    """The slab kernel forms (0..2) are switched off in this build: the
+    per-lane form (3) and autotune (-1, which then keeps the per-lane form)
+    are accepted, a slab form is refused with an error and no device work."""
+    L = abi.lib()
+    assert L.brams_chem_set_form(0) == -6
+    assert b"switched off" in L.brams_chem_last_error()
+    assert L.brams_chem_set_form(7) == -2
+    assert L.brams_chem_set_form(3) == 0
+    assert L.brams_chem_set_form(-1) == 0

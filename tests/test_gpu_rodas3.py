@@ -34,6 +34,9 @@ def test_16x16x10_device_matches_oracle(seed, tol):
     assert rc == 0 and orc == 0
     assert st.mask_nnz == ost.mask_nnz
     assert st.forced_blocks == ost.forced_blocks
+    assert st.attempts == ost.attempts and st.substeps == ost.substeps
+    assert st.threshold_pivots == ost.threshold_pivots
+    assert st.order_threshold_pivots == ost.order_threshold_pivots == 0
     assert rep["nfail"] == 0, rep
     np.testing.assert_allclose(dt_gpu, hs.last_dt, rtol=1e-6)
 
@@ -136,3 +139,4 @@ def test_split_methods_match_oracle(method, ndyn):
         worst = max(worst, float((np.abs(a - b) / (np.abs(b) + 1e-30)).max()))
         assert not bad.any(), (s, int(bad.sum()))
     print("worst rel %.3g" % worst)
+

@@ -187,3 +187,59 @@ def test_fallback_is_deterministic():
         sur.step()                                      # starts from that memory
         out.append(ds.sc_p.cpu().numpy().copy())
     assert np.array_equal(out[0], out[1])
+
+
+def test_fallback_failure_is_reported():
+    """A cell whose temperature is not finite fails the surrogate's checks
+    (non-finite result), goes to the RODAS3 fallback, and the fallback block
+    ends at once on its non-finite error norm (the bounded failure path of
+    the kernel).  brams_sur_step must report it: fallback_failed_blocks and
+    a positive return value, not a silent success."""
+    import torch
+    print("case sur_step fallback failure, 16x16x4 seed 11, one NaN temperature", flush=True)
+    wl, model, ds, sur, atol, rtol = _setup(targ_sd=1.0, spread_tol=1e30)
+    g = wl["grid"]
+    o = 2 + g.m1 * (g.ia + 3) + g.m1 * g.m2 * (g.ja + 2)      # an interior (k,i,j)
+    ds.theta[o] = float("nan")
+    torch.cuda.synchronize()
+    s = sur.step()
+    print("fallback cells %d, failed blocks %d" % (s.fallback_cells, s.fallback_failed_blocks))
+    assert s.fallback_cells >= 1
+    assert s.reasons[0] >= 1                       # flagged as non-finite
+    assert s.fallback_failed_blocks >= 1
+    from brams_amd.surrogate import _bind
+    import ctypes as C
+    s2 = type(s)()
+    ds.sc_p.copy_(ds.sc_p0)
+    rc = _bind().brams_sur_step(C.byref(ds.args), C.byref(sur.cmodel), C.byref(s2))
+    assert rc == s2.fallback_failed_blocks >= 1
+
+
+def test_grids_called_in_turn_keep_their_fallback_memory():
+    """Two grids called alternately, as nested grids are (one call per
+    grid), keep their own device state: grid A's second surrogate step,
+    whose fallback starts from the per-cell step memory of its first, is
+    bit-identical whether or not grid B ran in between."""
+    from brams_amd.chem import DeviceStep
+    from brams_amd.surrogate import SurrogateStep
+    from brams_amd.workload import Grid, make_workload, tolerances
+    print("case two grids in turn, 16x16x4 and 12x12x4 (dt 60 s), all cells to the fallback", flush=True)
+    atol, rtol = tolerances("reference")
+    wa = make_workload(Grid(16, 16, 4), seed=11, sza="split")
+    wb = make_workload(Grid(12, 12, 4, dtlt=60.0), seed=12, sza="split")
+    ma, mb = _model(wa, 5, 1.0), _model(wb, 6, 1.0)
+    ma.spread_tol = mb.spread_tol = 0.0
+    ma.n_tol = ma.c_tol = mb.n_tol = mb.c_tol = 1e30
+    da, db = DeviceStep(wa, atol, rtol), DeviceStep(wb, atol, rtol)
+    sa, sb = SurrogateStep(ma, da), SurrogateStep(mb, db)
+    sa.reset()
+    sa.step()
+    sa.step()
+    alone = da.sc_p.cpu().numpy().copy()
+    da.reset()
+    db.reset()
+    sa.reset()
+    sa.step()
+    sb.step()
+    sa.step()
+    assert np.array_equal(da.sc_p.cpu().numpy(), alone)

@@ -167,19 +167,110 @@ def _step(seed, tol, **kw):
     return wl, O.run_oracle(wl, atol, rtol, **kw)
 
 
+STATES = [("random", None, 0), ("random", None, 1), ("split", 3.0, 0), ("split_night_first", 3.0, 0)]
+
+
 @pytest.mark.ref
 @pytest.mark.parametrize("tol", ["reference", "baseline"])
-def test_step_with_reference_sparse(tol):
-    """16x16x10 workload: the oracle's fixed-order LU == the reference's Sparse1.3."""
-    wl, (hs, st, mask, rc) = _step(0, tol)
-    _, (hr, sr, maskr, rcr) = _step(0, tol, reference_sparse=True)
+@pytest.mark.parametrize("sza,plume,seed", STATES)
+def test_step_with_reference_sparse(sza, plume, seed, tol):
+    """16x16x10 workloads (random SZA; split SZA with a plume and the mask
+    cell in daylight; the same mirrored, mask cell at night): the oracle's
+    fixed-order LU against the reference's own Sparse1.3, which re-derives a
+    Markowitz order from cell 1 of block 1 on every block-1 attempt
+    (chem_spack_rodas3_dyndt.f90:470-517).  Same mask, attempts and
+    sub-steps; outputs within the parity bar (in practice bit-identical
+    REAL values in all but a few cells)."""
+    wl = make_workload(Grid(16, 16, 10), seed=seed, sza=sza, plume_sigma=plume)
+    atol, rtol = tolerances(tol)
+    hs, st, mask, rc = O.run_oracle(wl, atol, rtol)
+    hr, sr, maskr, rcr = O.run_oracle(wl, atol, rtol, reference_sparse=True)
     assert rc == 0 and rcr == 0
     assert st.attempts == sr.attempts and st.substeps == sr.substeps
     assert np.array_equal(mask, maskr)
-    a, b = np.stack(hs.sc_p), np.stack(hr.sc_p)
+    a, b = np.stack(hs.sc_p).astype(np.float64), np.stack(hr.sc_p).astype(np.float64)
     rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
     assert rel.max() <= 1e-6
-    np.testing.assert_allclose(hs.last_dt, hr.last_dt, rtol=1e-12)
+    assert np.mean(a == b) > 0.9999
+    np.testing.assert_allclose(hs.last_dt, hr.last_dt, rtol=1e-11)
+
+
+@pytest.mark.parametrize("tol", ["reference", "baseline"])
+@pytest.mark.parametrize("sza,plume,seed", STATES)
+def test_fixed_order_passes_sparse_threshold(sza, plume, seed, tol):
+    """Every pivot of the fixed order passes Sparse1.3's threshold test
+    (|pivot| > 1e-3 x largest other entry of its reduced column,
+    spFactor.c:202-203, spAllocate.c:191) in every cell and attempt, and in
+    particular on cell 1 of block 1, the matrix the reference orders on."""
+    wl = make_workload(Grid(16, 16, 10), seed=seed, sza=sza, plume_sigma=plume)
+    atol, rtol = tolerances(tol)
+    hs, st, mask, rc = O.run_oracle(wl, atol, rtol)
+    assert rc == 0
+    assert st.threshold_pivots == 0 and st.order_threshold_pivots == 0
+
+
+def test_threshold_count_catches_a_small_pivot():
+    """The count is live: pivot J of the fixed order set to 0.5 under column
+    entries of 1000 fails the rule (1000 x 1e-3 >= 0.5) exactly once; 0.5
+    under entries of 100 passes."""
+    import ctypes as C
+    perm, _ = O.oracle_lu_order()
+    lib = O.oracle()
+    lib.ora_lu_threshold_fails.argtypes = [O._dp, C.POINTER(C.c_uint8)]
+    mask = np.ones(47 * 47, np.uint8)
+
+    def count(J, below, pivot):
+        a = np.eye(47) * 2.0
+        a[perm[J], perm[J]] = pivot
+        for I in range(J + 1, 47):
+            a[perm[I], perm[J]] = below         # entries outside the filled pattern are ignored
+        a = np.ascontiguousarray(a)
+        return lib.ora_lu_threshold_fails(O.ptr(a), mask.ctypes.data_as(C.POINTER(C.c_uint8)))
+
+    assert count(13, 0.0, 0.5) == 0
+    assert count(13, 1000.0, 0.5) == 1
+    assert count(13, 100.0, 0.5) == 0
+    assert count(40, 1000.0, 0.5) == 1
+
+
+@pytest.mark.ref
+def test_reference_sparse_order_depends_on_pattern_only():
+    """Sparse1.3's Markowitz order (spOrderAndFactor, spFactor.c:173-300) of
+    cell-1 matrices taken from many cells, states and step sizes is one
+    order per sparsity pattern: the daylight pattern and the night pattern
+    (photolysis entries zero) each give a single order.  The order is thus a
+    function of the mask, which the fixed-order LU exploits."""
+    import ctypes as C
+    R = O.ref()
+    rng = np.random.default_rng(0)
+    orders = {}
+    for seed, sza in [(0, "random"), (1, "split"), (2, "split_night_first")]:
+        wl = make_workload(Grid(16, 16, 10), seed=seed, sza=sza, plume_sigma=None if sza == "random" else 3.0)
+        g = wl["grid"]
+        for cell in rng.integers(0, g.ncell, 12):
+            cell = int(cell)
+            y, T, p, _ = O.gather_cell(wl, cell)
+            k, r = 2 + cell % g.nz, cell // g.nz
+            i, j = 2 + r % g.nx, 2 + r // g.nx
+            jph = np.asarray(wl["jphoto"][:, k - 1, i - 1, j - 1])
+            rk = O.oracle_kinetic([T], [p], [float(wl["rv"][k - 1, i - 1, j - 1])], [jph])[0]
+            J = O.oracle_jacobian(rk[None], y[None])[0]
+            for dt in (120.0, 3.0):
+                A = -J + np.eye(47) * (1.0 / (0.5 * dt))
+                ii, jj = np.nonzero(A)
+                ipos = np.ascontiguousarray(ii + 1, np.int32)
+                jpos = np.ascontiguousarray(jj + 1, np.int32)
+                vals = np.ascontiguousarray(A[ii, jj])
+                elem = (C.c_long * len(ii))()
+                h = R.ref_sp_create(47, len(ii), O.ptr(ipos, O._ip), O.ptr(jpos, O._ip), elem)
+                assert R.ref_sp_load_factor(h, len(ii), elem, O.ptr(vals)) == 0
+                rm = np.zeros(47, np.int32); cm = np.zeros(47, np.int32); fi = np.zeros(1, np.int32)
+                R.ref_sp_order(h, O.ptr(rm, O._ip), O.ptr(cm, O._ip), O.ptr(fi, O._ip))
+                R.ref_sp_destroy(h)
+                assert np.array_equal(rm, cm)                   # diagonal pivoting
+                orders.setdefault((ii * 47 + jj).tobytes(), set()).add(tuple(rm))
+    assert len(orders) >= 2                                     # day and night patterns seen
+    assert all(len(v) == 1 for v in orders.values())
 
 
 def _sulphur(wl, sc):
