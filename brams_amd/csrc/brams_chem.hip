@@ -128,6 +128,50 @@ struct State {
 };
 State g;
 
+// Bookkeeping of brams_chem_rodas3_dyndt_device_noalloc (block starts, mask
+// words, per-block counters, failure count) lives in one file-scope device
+// array instead of allocated buffers.  Every device holds its own copy for
+// the life of the process; on the host, `scratch` names a host-side stand-in,
+// so only the device address taken per device by scratch_address() is ever
+// passed to kernels or copies.  Every region is fully written by each call
+// before it is read (nothing has to start at zero), and calls that overlap on
+// one device race on it.
+constexpr int kNoAllocMaxBlocks = 32768;   // 2 097 152 cells at 64 cells per block
+constexpr size_t na_round(size_t x) { return (x + 255) / 256 * 256; }
+constexpr size_t kNaStart = 0;                                                           // int[max + 1]
+constexpr size_t kNaMask = na_round(kNaStart + sizeof(int) * (kNoAllocMaxBlocks + 1));    // unsigned[words]
+constexpr size_t kNaAttempts = na_round(kNaMask + sizeof(unsigned) * MECH_MASK_WORDS);     // int[max]
+constexpr size_t kNaSubsteps = na_round(kNaAttempts + sizeof(int) * kNoAllocMaxBlocks);    // int[max]
+constexpr size_t kNaFailed = na_round(kNaSubsteps + sizeof(int) * kNoAllocMaxBlocks);      // int[max]
+constexpr size_t kNaForced = na_round(kNaFailed + sizeof(int) * kNoAllocMaxBlocks);        // float[max]
+constexpr size_t kNaThresh = na_round(kNaForced + sizeof(float) * kNoAllocMaxBlocks);      // int[max]
+constexpr size_t kNaOrderThresh = na_round(kNaThresh + sizeof(int) * kNoAllocMaxBlocks);   // int[1]
+constexpr size_t kNaCount = na_round(kNaOrderThresh + sizeof(int));                        // int[1]
+constexpr size_t kNoAllocBytes = na_round(kNaCount + sizeof(int));
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+unsigned char* g_scratch[kMaxDevices];   // device address of scratch per device (null: not taken)
+
+// the device address of scratch on the current device, taken on the first
+// call there (hipGetSymbolAddress with that device current)
+int scratch_address(unsigned char** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(-2, "device %d is beyond the %d devices the no-allocation entry serves", dev, kMaxDevices);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(-100 - (int)e, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+
 // AllocIndex, chem_spack_utils.f90:93-125
 int partition(int block_size, long long ncell, std::vector<int>& sizes) {
   if (ncell <= 0 || block_size <= 0) return -1;
@@ -463,6 +507,8 @@ int brams_chem_init(int device) {
     if (!e) HIP_TRY(hipEventCreate(&e));
   g.n_cu = prop.multiProcessorCount;
   HIP_TRY(g.qcount.ensure(sizeof(int) * 64));
+  unsigned char* na = nullptr;
+  if (int rc = scratch_address(&na)) return rc;
   g.init = true;
   return 0;
 }
@@ -484,13 +530,9 @@ int brams_chem_alloc_index(int32_t block_size, int32_t m1, int32_t ia, int32_t i
 
 namespace {
 
-// validate device arguments and fill the kernel parameters (no launch)
-int prepare_device(const brams_chem_dev_args* a, brams::StepParams& p, long long& ncell) {
+// argument checks shared by the device entries (no device work)
+int check_device_args(const brams_chem_dev_args* a) {
   if (!a) return fail(-2, "null args");
-  if (!g.init) {
-    int rc = brams_chem_init(0);
-    if (rc) return rc;
-  }
   int rc = check_common(a->m1, a->m2, a->m3, a->ia, a->iz, a->ja, a->jz, a->n_dyn_chem,
                         a->split_method, a->photoj_method, a->dtlt);
   if (rc) return rc;
@@ -501,10 +543,11 @@ int prepare_device(const brams_chem_dev_args* a, brams::StepParams& p, long long
     return fail(-2, "PARALLEL with N_DYN_CHEM=1 needs sc_t");
   if (a->split_method == BRAMS_SPLIT_PARALLEL && a->n_dyn_chem > 1 && !a->sc_t_dyn)
     return fail(-2, "PARALLEL with N_DYN_CHEM>1 needs sc_t_dyn");
-  if ((rc = ensure_grid(a->m1, a->m2, a->m3, a->ia, a->iz, a->ja, a->jz))) return rc;
-  ncell = g.cur->ncell;
-  const int nob = (int)g.cur->sizes.size();
-  if (a->nob != nob) return fail(-2, "nob %d does not match the AllocIndex partition (%d)", a->nob, nob);
+  return 0;
+}
+
+// the kernel parameters that come from the caller's arguments
+void fill_params(const brams_chem_dev_args* a, brams::StepParams& p) {
   p = brams::StepParams{};
   p.m1 = a->m1; p.m2 = a->m2; p.m3 = a->m3;
   p.ia = a->ia; p.iz = a->iz; p.ja = a->ja; p.jz = a->jz;
@@ -518,6 +561,22 @@ int prepare_device(const brams_chem_dev_args* a, brams::StepParams& p, long long
   p.n3d = (long long)a->m1 * a->m2 * a->m3;
   p.last_dt = a->last_accepted_dt;
   p.atol = a->atol; p.rtol = a->rtol;
+}
+
+// validate device arguments and fill the kernel parameters (no launch)
+int prepare_device(const brams_chem_dev_args* a, brams::StepParams& p, long long& ncell) {
+  if (!a) return fail(-2, "null args");
+  if (!g.init) {
+    int rc = brams_chem_init(0);
+    if (rc) return rc;
+  }
+  int rc = check_device_args(a);
+  if (rc) return rc;
+  if ((rc = ensure_grid(a->m1, a->m2, a->m3, a->ia, a->iz, a->ja, a->jz))) return rc;
+  ncell = g.cur->ncell;
+  const int nob = (int)g.cur->sizes.size();
+  if (a->nob != nob) return fail(-2, "nob %d does not match the AllocIndex partition (%d)", a->nob, nob);
+  fill_params(a, p);
   return 0;
 }
 
@@ -533,6 +592,56 @@ int brams_chem_rodas3_dyndt_device(const brams_chem_dev_args* a) {
   hipStream_t st = (hipStream_t)a->stream;
   if ((rc = launch_step(p, st))) return rc;
   return collect_stats(st);
+}
+
+// The same step with no allocation: the bookkeeping sits in `scratch`, the
+// blocks run in partition order through the same kernels (block 1, then the
+// 65..127-cell blocks on 128 lanes, then the 64-cell blocks).
+int brams_chem_rodas3_dyndt_device_noalloc(const brams_chem_dev_args* a) {
+  int rc = check_device_args(a);
+  if (rc) return rc;
+  const long long ncell = (long long)(a->m1 - 2) * (a->iz - a->ia + 1) * (a->jz - a->ja + 1);
+  const long long nobl = ncell <= 64 ? 1 : ncell / 64;
+  if (nobl > kNoAllocMaxBlocks)
+    return fail(-7, "%lld blocks exceed the no-allocation limit of %d blocks (%lld cells)", nobl,
+                kNoAllocMaxBlocks, 64LL * kNoAllocMaxBlocks);
+  const int nob = (int)nobl;
+  if (a->nob != nob) return fail(-2, "nob %d does not match the AllocIndex partition (%d)", a->nob, nob);
+  // AllocIndex sizes: base + (b < r) cells
+  const long long rest = ncell <= 64 ? 0 : ncell - 64LL * nob;
+  const int base = ncell <= 64 ? (int)ncell : 64 + (int)(rest / nob);
+  const int r = (int)(rest % nob);
+  const int size0 = base + (r > 0 ? 1 : 0);
+  if (size0 > 128) return fail(-3, "block of %d cells exceeds 128", size0);
+  const int nbig = base > 64 ? nob : (size0 > 64 ? r : 0);   // blocks of more than 64 cells
+  unsigned char* s = nullptr;
+  if ((rc = scratch_address(&s))) return rc;
+  brams::StepParams p;
+  fill_params(a, p);
+  int* starts = reinterpret_cast<int*>(s + kNaStart);
+  p.block_start = starts;
+  p.mask = reinterpret_cast<unsigned*>(s + kNaMask);
+  p.blk_attempts = reinterpret_cast<int*>(s + kNaAttempts);
+  p.blk_substeps = reinterpret_cast<int*>(s + kNaSubsteps);
+  p.blk_failed = reinterpret_cast<int*>(s + kNaFailed);
+  p.blk_forced = reinterpret_cast<float*>(s + kNaForced);
+  p.blk_thresh = reinterpret_cast<int*>(s + kNaThresh);
+  p.order_thresh = reinterpret_cast<int*>(s + kNaOrderThresh);
+  int* count = reinterpret_cast<int*>(s + kNaCount);
+  hipStream_t st = (hipStream_t)a->stream;
+  hipLaunchKernelGGL(brams::partition_starts, dim3((unsigned)((nob + 256) / 256)), dim3(256), 0, st, nob,
+                     base, r, starts);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(brams::launch_rodas3(size0 > 64 ? 128 : 64, true, 1, p, 0, st));
+  if (nbig > 1) HIP_TRY(brams::launch_rodas3(128, false, (unsigned)(nbig - 1), p, 1, st));
+  const int first64 = nbig > 1 ? nbig : 1;
+  if (nob > first64) HIP_TRY(brams::launch_rodas3(64, false, (unsigned)(nob - first64), p, first64, st));
+  hipLaunchKernelGGL(brams::count_nonzero, dim3(1), dim3(256), 0, st, p.blk_failed, nob, count);
+  HIP_TRY(hipGetLastError());
+  int nfail = 0;
+  HIP_TRY(hipMemcpyAsync(&nfail, count, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return nfail;
 }
 
 // ------------------------------------------------------------------ surrogate

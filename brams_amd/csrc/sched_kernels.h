@@ -102,4 +102,26 @@ __global__ void cell_history_update(const int* __restrict__ list, int nfb, const
   cell_dt[list[i]] = fb_dt[i / 64];
 }
 
+// AllocIndex block starts in closed form (chem_spack_utils.f90:93-125): the
+// remainder ncell - 64*nob is spread one cell at a time over the blocks from
+// the first on, so block b holds base + (b < r) cells with base = 64 + the
+// whole rounds and r the last partial round.  block_start[0..nob].
+__global__ void partition_starts(int nob, int base, int r, int* __restrict__ block_start) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b <= nob) block_start[b] = b * base + (b < r ? b : r);
+}
+
+// out[0] = how many of flags[0..n) are nonzero; one workgroup of 256
+__global__ __launch_bounds__(256) void count_nonzero(const int* __restrict__ flags, int n,
+                                                     int* __restrict__ out) {
+  __shared__ int part[4];
+  int v = 0;
+  for (int i = threadIdx.x; i < n; i += 256) v += flags[i] != 0 ? 1 : 0;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = part[0] + part[1] + part[2] + part[3];
+}
+
 }  // namespace brams

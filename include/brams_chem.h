@@ -138,6 +138,23 @@ int brams_chem_rodas3_dyndt(const brams_chem_args *args);
 int brams_chem_rodas3_dyndt_device(const brams_chem_dev_args *args);
 
 /*
+ * No counterpart in the reference.  brams_chem_rodas3_dyndt_device for
+ * callers that cannot allocate device memory (code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the same step
+ * through the same kernels, with the per-call bookkeeping (block starts,
+ * mask words, per-block counters, the failure count) in a fixed device
+ * array of the library (about 0.8 MB, one copy per device, held until the
+ * process ends) instead of allocated buffers.  It calls no allocator.
+ * At most 32768 blocks (2 097 152 cells); a larger grid is refused (-7)
+ * before any device work.  Calls on one device share that array, so calls
+ * that overlap on one device race.  Blocks run in partition order;
+ * brams_chem_get_stats and brams_chem_block_status do not report these calls.
+ * Synchronises the stream; returns the number of failed blocks (>= 0) or
+ * an error code (< 0).
+ */
+int brams_chem_rodas3_dyndt_device_noalloc(const brams_chem_dev_args *args);
+
+/*
  * Block partition of one grid, AllocIndex (chem_spack_utils.f90:39-183):
  * interior cells in j->i->k order cut into floor(N/block_size) blocks, the
  * remainder spread +1 over the first blocks.  Writes block_end[0..nob-1] when

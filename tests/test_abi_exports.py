@@ -82,3 +82,22 @@ def test_slab_forms_are_refused_while_switched_off():
     assert L.brams_chem_set_form(7) == -2
     assert L.brams_chem_set_form(3) == 0
     assert L.brams_chem_set_form(-1) == 0
+
+
+def test_noalloc_entry_refuses_grids_beyond_its_limit():
+    """brams_chem_rodas3_dyndt_device_noalloc serves at most 32768 blocks;
+    a larger grid (300x300x40: 56 250 blocks) is refused with -7 before any
+    device work (the pointers are never dereferenced)."""
+    L = abi.lib()
+    L.brams_chem_rodas3_dyndt_device_noalloc.argtypes = [C.POINTER(abi.ChemDevArgs)]
+    g = Grid(300, 300, 40)
+    a = abi.ChemDevArgs()
+    a.m1, a.m2, a.m3, a.ia, a.iz, a.ja, a.jz = g.m1, g.m2, g.m3, g.ia, g.iz, g.ja, g.jz
+    a.dtlt, a.n_dyn_chem = 120.0, 1
+    dummy = 0x1000
+    a.pp = a.pi0 = a.theta = a.rv = a.jphoto = a.weight = a.transported = dummy
+    a.sc_p = a.last_accepted_dt = a.atol = a.rtol = dummy
+    a.nob = g.ncell // 64
+    assert a.nob == 56250
+    assert L.brams_chem_rodas3_dyndt_device_noalloc(C.byref(a)) == -7
+    assert b"no-allocation limit" in L.brams_chem_last_error()
