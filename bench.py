@@ -97,7 +97,9 @@ def cpu_baseline(warm_wl, cold_wl, atol, rtol, warm_every=4, cold_every=40):
         out[tag] = {"value": cells / dt, "cells": cells, "blocks": len(blocks) + 1, "wall_s": dt,
                     "cell_attempts": int(st.cell_attempts)}
     w, c = out["warm"], out["cold"]
+    calib = calibrate_oracle(warm_wl, atol, rtol, cores)
     return {"value": w["value"], "unit": "cell-steps/s", "cores": cores, "kind": "port",
+            "calibration": calib,
             "sample": "oracle (C restatement of chem_rodas3_dyndt with the reference's kernels' "
                       "arithmetic, gcc -O2 -ffp-contract=off, pthreads over blocks) on block 1 and every "
                       "%dth block of the warm state (%d blocks, %d cells x 1 step, carried last_accepted_dt, "
@@ -108,15 +110,46 @@ def cpu_baseline(warm_wl, cold_wl, atol, rtol, warm_every=4, cold_every=40):
                            "%d cell-attempts)" % (cold_every, c["cells"], c["wall_s"], c["cell_attempts"])}
 
 
+def calibrate_oracle(wl, atol, rtol, cores, every=200):
+    """The oracle against the reference's own compiled pieces: the same block
+    sample (block 1 and every `every`-th block of the warm state) once with the
+    oracle's LU and once with its LU routed through the reference's compiled
+    Sparse1.3 (spOrderAndFactor / spFactor / spSolve on the value-ordered
+    matrix, oracle/_ref).  Equal outputs and a comparable time show that the
+    port is not a slowed-down restatement."""
+    import numpy as np
+    import oracle_lib
+    nob = len(wl["block_end"])
+    blocks = np.arange(every, nob, every, dtype=np.int32)
+    res = {}
+    try:
+        for tag, ext in (("oracle_lu", False), ("reference_sparse", True)):
+            oracle_lib.use_reference_sparse(ext)
+            t0 = time.perf_counter()
+            hs, st, mask, rc = oracle_lib.run_oracle_blocks(wl, atol, rtol, blocks, nthreads=cores)
+            res[tag] = (time.perf_counter() - t0, [np.asarray(x).copy() for x in hs.sc_p], rc)
+    except OSError as e:                                # oracle/_ref not built on this machine
+        return {"error": "reference pieces not available: %s" % e}
+    finally:
+        oracle_lib.use_reference_sparse(False)
+    same = all(np.array_equal(a, b) for a, b in zip(res["oracle_lu"][1], res["reference_sparse"][1]))
+    return {"sample": "block 1 and every %dth block of the warm state (%d blocks)" % (every, len(blocks) + 1),
+            "oracle_lu_s": res["oracle_lu"][0], "reference_sparse_s": res["reference_sparse"][0],
+            "outputs_identical": bool(same)}
+
+
 def rodas3_flops(cells, cell_substeps, cell_attempts):
     f_call, f_sub, f_att = flop_model()
     return f_call * cells + f_sub * cell_substeps + f_att * cell_attempts
 
 
 def bench_nested(args):
-    """Three grids per step, each its own chem_rodas3_dyndt call (as the
-    reference drives nested grids): cell-steps/s over the 2.08M cells, every
-    step from the same seeded states (cold steps)."""
+    """Three nested grids per step from the same seeded states (cold steps):
+    first one chem_rodas3_dyndt call per grid, as the reference drives nested
+    grids (chemistry.f90:313-320), then all three in one pass
+    (brams_chem_rodas3_dyndt_device_grids).  value: cell-steps/s of the
+    one-pass form over the 2.08M cells; both forms' times are in the line."""
+    import ctypes as C
     import torch
     from brams_amd import abi
     from brams_amd.chem import DeviceStep
@@ -126,9 +159,12 @@ def bench_nested(args):
     steps = [DeviceStep(make_workload(g, seed=0, sza="split", plume_sigma=20.0 * g.nx / 200.0,
                                       plume_factor=50.0), atol, rtol) for g in grids]
     ncell = sum(g.ncell for g in grids)
+    L = abi.lib()
+    L.brams_chem_rodas3_dyndt_device_grids.argtypes = [C.POINTER(abi.ChemDevArgs), C.c_int32]
+    arr = (abi.ChemDevArgs * len(steps))(*[st.args for st in steps])
     print("bench: nested grids, %d cells" % ncell, file=sys.stderr, flush=True)
 
-    def one_step():
+    def calls():
         out = []
         for st in steps:
             st.reset()
@@ -137,18 +173,30 @@ def bench_nested(args):
             out.append((s.kernel_ms, s.cell_attempts, s.cells))
         return out
 
-    for _ in range(args.warmup):
-        one_step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    per = []
-    for i in range(args.steps):
-        per.append(one_step())
-        print("bench: nested step %d: %s ms" % (i, ", ".join("%.1f" % x[0] for x in per[-1])),
-              file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    f_call, f_sub, f_att = flop_model()
+    def one_pass():
+        for st in steps:
+            st.reset()
+        rc = L.brams_chem_rodas3_dyndt_device_grids(arr, len(steps))
+        if rc < 0:
+            raise RuntimeError(L.brams_chem_last_error().decode())
+        s = abi.stats()
+        return [(s.kernel_ms, s.cell_attempts, s.cells)]
+
+    timed = {}
+    for name, fn in (("calls", calls), ("pass", one_pass)):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        per = []
+        for i in range(args.steps):
+            per.append(fn())
+            print("bench: nested %s step %d: %s ms" % (name, i, ", ".join("%.1f" % x[0] for x in per[-1])),
+                  file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        timed[name] = (time.perf_counter() - t0, per)
+    wall, per = timed["pass"]
+    wall_calls, per_calls = timed["calls"]
     line = {
         "metric": "chem cell-steps/s, surrogate+fallback vs FP64 Rosenbrock, at <=1% rel err",
         "value": ncell * args.steps / wall, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -157,9 +205,12 @@ def bench_nested(args):
         "data": "synthetic (seeded Philox generator, brams_amd/workload.py)",
         "config": {"workload": "RELACS_TUV RODAS3 cold step on three nested grids: 200x200x40 (dt 120 s), "
                                "98x98x40 (dt 60 s), 50x50x40 (dt 30 s), SZA split, plume, seed 0",
-                   "path": "plain FP64 RODAS3 (GPU), one call per grid", "tolerances": args.tol,
-                   "cells": ncell, "kernel_ms_per_grid": [sum(p[g][0] for p in per) / args.steps for g in range(3)],
-                   "attempts_per_cell_per_grid": [per[-1][g][1] / per[-1][g][2] for g in range(3)]},
+                   "path": "plain FP64 RODAS3 (GPU), the three grids in one pass", "tolerances": args.tol,
+                   "cells": ncell, "one_pass_ms_per_step": 1e3 * wall / args.steps,
+                   "one_call_per_grid_ms_per_step": 1e3 * wall_calls / args.steps,
+                   "kernel_ms_per_grid_one_call_each": [sum(p[g][0] for p in per_calls) / args.steps
+                                                        for g in range(3)],
+                   "attempts_per_cell_per_grid": [per_calls[-1][g][1] / per_calls[-1][g][2] for g in range(3)]},
         "roofline": None,
     }
     print(json.dumps(line), flush=True)
@@ -177,7 +228,7 @@ def main():
     ap.add_argument("--tol", default="baseline", choices=["baseline", "reference"])
     ap.add_argument("--path", default="surrogate", choices=["surrogate", "fp64", "nested"],
                     help="nested: the plain FP64 step on three nested grids (200x200x40 at dt 120 s, "
-                         "98x98x40 at 60 s, 50x50x40 at 30 s), one call per grid")
+                         "98x98x40 at 60 s, 50x50x40 at 30 s), one call per grid and in one pass")
     ap.add_argument("--model", default=os.path.join(ROOT, "brams_amd", "models", "relacs_tuv_surrogate.npz"))
     ap.add_argument("--spread-tol", type=float, default=None, help="override the model's spread check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")

@@ -6,6 +6,7 @@
   4. make -C oracle                 -> oracle/liboracle.so (+ oracle/_ref when the
                                        reference tree is present)
 """
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -27,16 +28,22 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
 # in outputs and attempt counts; parity at full size is checked by the
 # 1.6M-cell tests, which report accept/reject flips separately.
 OBJ = os.path.join(PKG, "_build")
-# translation units of libbrams_chem.so; the per-lane RODAS3 kernels (about
-# fifteen minutes of compile) live in rodas3_launch.hip alone, the slab
-# kernels (two minutes) in rodas3_slab.hip
-HIP_UNITS = ["rodas3_launch.hip", "rodas3_slab.hip", "brams_chem.hip", "surrogate.hip"]
+# translation units of libbrams_chem.so.  The per-lane RODAS3 kernels take
+# about two minutes of compile each, so they are spread over three units
+# (rodas3_launch*.hip, rodas3_grids.hip) that compile in parallel; the slab
+# kernels (two minutes) live in rodas3_slab.hip.
+HIP_UNITS = ["rodas3_launch.hip", "rodas3_launch_b.hip", "rodas3_grids.hip", "rodas3_slab.hip",
+             "brams_chem.hip", "surrogate.hip"]
+_RODAS3_DEPS = ["rodas3_kernel.h", "rodas3_launch.h", "mech_families.h", "generated/relacs_tuv_gen.h"]
 UNIT_DEPS = {
-    "rodas3_launch.hip": ["rodas3_kernel.h", "rodas3_launch.h", "mech_families.h",
-                          "generated/relacs_tuv_gen.h"],
+    "rodas3_launch.hip": _RODAS3_DEPS,
+    "rodas3_launch_b.hip": _RODAS3_DEPS,
+    "rodas3_grids.hip": _RODAS3_DEPS,
     "rodas3_slab.hip": ["rodas3_slab.h", "rodas3_slab_launch.h", "rodas3_kernel.h", "mech_families.h",
                         "generated/relacs_tuv_gen.h", "generated/relacs_tuv_slab.h"],
 }
+# parallel compiles (each needs a few GB of host memory)
+HIP_JOBS = int(os.environ.get("BRAMS_HIP_JOBS", "4"))
 REF_ROOT = os.environ.get("BRAMS_REFERENCE", "/root/reference")
 
 
@@ -61,7 +68,7 @@ def build_hip(force=True):
     """Compile each unit to an object under _build/ and link the library.
     force=False recompiles only units older than their source or a header."""
     os.makedirs(OBJ, exist_ok=True)
-    objs = []
+    objs, jobs = [], []
     for unit in HIP_UNITS:
         src = os.path.join(CSRC, unit)
         obj = os.path.join(OBJ, unit.replace(".hip", ".o"))
@@ -69,9 +76,17 @@ def build_hip(force=True):
         stale = (force or not os.path.exists(obj)
                  or os.path.getmtime(obj) < max(os.path.getmtime(f) for f in deps + [src]))
         if stale:
-            run([HIPCC] + HIP_FLAGS + ["-I", CSRC, "-c", "-o", obj + ".tmp", src])
-            os.replace(obj + ".tmp", obj)
+            jobs.append((src, obj))
         objs.append(obj)
+
+    def compile_unit(job):
+        src, obj = job
+        run([HIPCC] + HIP_FLAGS + ["-I", CSRC, "-c", "-o", obj + ".tmp", src])
+        os.replace(obj + ".tmp", obj)
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, HIP_JOBS)) as ex:
+        for f in [ex.submit(compile_unit, j) for j in jobs]:
+            f.result()
     tmp = LIB + ".tmp"
     run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
     os.replace(tmp, LIB)

@@ -71,9 +71,13 @@ struct DevBuf {
 
 // Kernel forms of the 64-cell partition blocks: the slab layouts
 // 0..kSlabVariants-1 (rodas3_slab.h) and the per-lane kernel
-// (rodas3_kernel.h).  All give bit-identical results.
-constexpr int kFormPerLane = brams::kSlabVariants;
-constexpr int kForms = brams::kSlabVariants + 1;
+// (rodas3_kernel.h) at 1, 2 and 4 waves per SIMD.  All give bit-identical
+// results; the autotuner keeps the fastest per grid.
+constexpr int kFormPerLane = brams::kSlabVariants;      // first per-lane form
+constexpr int kPerLaneWps[3] = {1, 2, 4};
+constexpr int kForms = brams::kSlabVariants + 3;
+// list (fallback) launches: per-lane kernel at 1 or 2 waves per SIMD
+constexpr int kListWps[2] = {1, 2};
 // The slab forms are switched off: on gfx950 their kernels did not
 // terminate, even for a one-block grid with the attempts capped at 300,
 // while a host build of the same block integration matches the oracle bit
@@ -96,7 +100,9 @@ struct GridState {
   std::vector<int> h_att, h_sub, h_fail, h_thresh;
   std::vector<float> h_forced;
   int form = -1;                 // kernel form of the 64-cell blocks (-1: not tuned yet)
-  float tune_ms[kForms] = {0, 0, 0, 0};
+  float tune_ms[kForms] = {};
+  int list_wps = 0;              // waves per SIMD of the fallback launches (0: not tuned yet)
+  float list_tune_ms[2] = {};
   bool same(int a1, int a2, int a3, int b1, int b2, int b3, int b4) const {
     return m1 == a1 && m2 == a2 && m3 == a3 && ia == b1 && iz == b2 && ja == b3 && jz == b4;
   }
@@ -120,6 +126,8 @@ struct State {
   int force_form = -1;           // brams_chem_set_form (-1: autotune per grid)
   // autotuning scratch (copies of the state a step writes)
   DevBuf tune_sc, tune_dt;
+  // several grids in one pass: per-grid kernel parameters and work lists
+  DevBuf multi_ps, multi_work;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evb = nullptr, evm = nullptr;
   hipEvent_t sev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   long long main_cells = 0;
@@ -292,10 +300,10 @@ int launch_main(int form, const brams::StepParams& p, int first64, int nob, cons
                 hipStream_t st) {
   const int n = nob - first64;
   if (n <= 0) return 0;
-  if (form == kFormPerLane) {
+  if (form >= kFormPerLane) {
     brams::StepParams pm = p;
     pm.block_order = order;
-    HIP_TRY(brams::launch_rodas3(64, false, (unsigned)n, pm, first64, st));
+    HIP_TRY(brams::launch_rodas3(64, false, (unsigned)n, pm, first64, st, kPerLaneWps[form - kFormPerLane]));
     return 0;
   }
   long long nslots = 0;
@@ -323,11 +331,13 @@ int launch_block1(const brams::StepParams& p, hipStream_t st) {
   return 0;
 }
 
-// Autotuner: on the first call of a grid, each kernel form integrates the
-// call's 64-cell blocks on a scratch copy of the state (block 1 has run on
-// it first, so the mask is the call's), timed with events; the fastest is
-// kept for the grid.  The forms give bit-identical results, so the choice
-// changes speed only.  The caller's arrays are not touched.
+// Autotuner: on the first call of a grid, each kernel form (the per-lane
+// kernel at 1, 2 and 4 waves per SIMD, and the slab layouts when they are
+// switched on) integrates the call's 64-cell blocks on a scratch copy of the
+// state (block 1 has run on it first, so the mask is the call's), timed
+// with events; the fastest is kept for the grid.  The forms give
+// bit-identical results, so the choice changes speed only.  The caller's
+// arrays are not touched.
 int autotune(brams::StepParams p, int first64, int nob, hipStream_t st) {
   GridState& gs = *g.cur;
   const size_t nsc = sizeof(float) * MECH_NS * (size_t)p.n3d;
@@ -352,8 +362,9 @@ int autotune(brams::StepParams p, int first64, int nob, hipStream_t st) {
   q.sc_p = sc1;
   q.sc_t = st1;
   q.last_dt = dt1;
-  int best = 0;
-  for (int f = 0; f < kForms; ++f) {
+  const int f0 = kSlabEnabled ? 0 : kFormPerLane;
+  int best = f0;
+  for (int f = f0; f < kForms; ++f) {
     HIP_TRY(hipMemcpyAsync(sc1, sc0, nsc, hipMemcpyDeviceToDevice, st));
     if (to_t) HIP_TRY(hipMemcpyAsync(st1, st0, nsc, hipMemcpyDeviceToDevice, st));
     HIP_TRY(hipMemcpyAsync(dt1, dt0, sizeof(double) * nob, hipMemcpyDeviceToDevice, st));
@@ -367,6 +378,35 @@ int autotune(brams::StepParams p, int first64, int nob, hipStream_t st) {
     if (ms < gs.tune_ms[best]) best = f;
   }
   gs.form = best;
+  return 0;
+}
+
+// The same for the surrogate's fallback launches: on the first fallback of
+// a grid both list forms (1 and 2 waves per SIMD) integrate the call's
+// list on a scratch copy of the state and step sizes; the faster is kept for
+// the grid.  q is the real launch's parameters (sc_p, last_dt of the list).
+int tune_list(const brams::StepParams& q, int nb, hipStream_t st) {
+  GridState& gs = *g.cur;
+  const size_t nsc = sizeof(float) * MECH_NS * (size_t)q.n3d;
+  HIP_TRY(g.tune_sc.ensure(nsc));
+  HIP_TRY(g.tune_dt.ensure(sizeof(double) * nb));
+  brams::StepParams t = q;
+  t.sc_p = g.tune_sc.as<float>();
+  t.last_dt = g.tune_dt.as<double>();
+  int best = 0;
+  for (int k = 0; k < 2; ++k) {
+    HIP_TRY(hipMemcpyAsync(t.sc_p, q.sc_p, nsc, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(t.last_dt, q.last_dt, sizeof(double) * nb, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipEventRecord(g.ev0, st));
+    HIP_TRY(brams::launch_rodas3_list((unsigned)nb, t, st, kListWps[k]));
+    HIP_TRY(hipEventRecord(g.ev1, st));
+    HIP_TRY(hipEventSynchronize(g.ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+    gs.list_tune_ms[k] = ms;
+    if (ms < gs.list_tune_ms[best]) best = k;
+  }
+  gs.list_wps = kListWps[best];
   return 0;
 }
 
@@ -388,7 +428,6 @@ int launch_step(brams::StepParams p, hipStream_t st) {
   while (nbig < nob && sz[nbig] > 64) ++nbig;
   const int first64 = nbig > 1 ? nbig : 1;
   int form = g.force_form >= 0 ? g.force_form : gs.form;
-  if (!kSlabEnabled) form = kFormPerLane;
   if (form < 0 && nob > first64) {
     int rc = autotune(p, first64, nob, st);
     if (rc) return rc;
@@ -415,7 +454,7 @@ int launch_step(brams::StepParams p, hipStream_t st) {
   }
   HIP_TRY(hipEventRecord(g.ev1, st));
   gs.blk_hist = true;
-  g.last_form = form;
+  g.last_form = nob > first64 ? form : -1;
   return 0;
 }
 
@@ -644,6 +683,114 @@ int brams_chem_rodas3_dyndt_device_noalloc(const brams_chem_dev_args* a) {
   return nfail;
 }
 
+// Several grids in one pass (the nested grids of a model step).  Block 1 of
+// every grid runs first, in one launch (each publishes its own grid's mask),
+// then the other 65..127-cell blocks of every grid, then the 64-cell blocks
+// of every grid, each in one launch.  From the second call on, the 64-cell
+// blocks launch longest first across the grids by their previous attempts.
+int brams_chem_rodas3_dyndt_device_grids(const brams_chem_dev_args* a, int32_t ngrids) {
+  constexpr int kMaxGrids = 16;
+  if (!a || ngrids < 1 || ngrids > kMaxGrids) return fail(-2, "ngrids must be 1..%d", kMaxGrids);
+  if (!g.init) {
+    int rc = brams_chem_init(0);
+    if (rc) return rc;
+  }
+  std::vector<GridState*> gss(ngrids);
+  std::vector<brams::StepParams> ps(ngrids);
+  std::vector<int> w1, w128, w64;
+  for (int i = 0; i < ngrids; ++i) {
+    int rc = check_device_args(&a[i]);
+    if (rc) return rc;
+    if (a[i].stream != a[0].stream) return fail(-2, "grid %d is on another stream than grid 1", i + 1);
+    if ((rc = ensure_grid(a[i].m1, a[i].m2, a[i].m3, a[i].ia, a[i].iz, a[i].ja, a[i].jz))) return rc;
+    GridState* gs = g.cur;
+    for (int j = 0; j < i; ++j)
+      if (gss[j] == gs) return fail(-2, "grids %d and %d have the same extents", j + 1, i + 1);
+    const int nob = (int)gs->sizes.size();
+    if (a[i].nob != nob)
+      return fail(-2, "grid %d: nob %d does not match the AllocIndex partition (%d)", i + 1, a[i].nob, nob);
+    if (nob > brams::kGridBlockMask) return fail(-2, "grid %d has too many blocks", i + 1);
+    gss[i] = gs;
+    brams::StepParams& p = ps[i];
+    fill_params(&a[i], p);
+    p.block_start = gs->block_start.as<int>();
+    p.mask = gs->mask.as<unsigned>();
+    p.blk_attempts = gs->attempts.as<int>();
+    p.blk_substeps = gs->substeps.as<int>();
+    p.blk_failed = gs->failed.as<int>();
+    p.blk_forced = gs->forced.as<float>();
+    p.blk_thresh = gs->thresh.as<int>();
+    p.order_thresh = gs->order_thresh.as<int>();
+    const int tag = i << brams::kGridShift;
+    int nbig = 0;
+    while (nbig < nob && gs->sizes[nbig] > 64) ++nbig;
+    w1.push_back(tag);
+    for (int b = 1; b < nbig; ++b) w128.push_back(tag | b);
+    for (int b = std::max(nbig, 1); b < nob; ++b) w64.push_back(tag | b);
+  }
+  bool hist = true;
+  for (GridState* gs : gss) hist = hist && gs->blk_hist && (int)gs->h_att.size() == (int)gs->sizes.size();
+  if (hist)
+    std::stable_sort(w64.begin(), w64.end(), [&](int x, int y) {
+      return gss[x >> brams::kGridShift]->h_att[x & brams::kGridBlockMask] >
+             gss[y >> brams::kGridShift]->h_att[y & brams::kGridBlockMask];
+    });
+  hipStream_t st = (hipStream_t)a[0].stream;
+  const size_t nw = w1.size() + w128.size() + w64.size();
+  HIP_TRY(g.multi_ps.ensure(sizeof(brams::StepParams) * ngrids));
+  HIP_TRY(g.multi_work.ensure(sizeof(int) * nw));
+  std::vector<int> work;
+  work.reserve(nw);
+  work.insert(work.end(), w1.begin(), w1.end());
+  work.insert(work.end(), w128.begin(), w128.end());
+  work.insert(work.end(), w64.begin(), w64.end());
+  const brams::StepParams* dps = g.multi_ps.as<brams::StepParams>();
+  const int* dw = g.multi_work.as<int>();
+  HIP_TRY(hipMemcpyAsync(g.multi_ps.p, ps.data(), sizeof(brams::StepParams) * ngrids, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(g.multi_work.p, work.data(), sizeof(int) * nw, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipEventRecord(g.ev0, st));
+  HIP_TRY(brams::launch_rodas3_grids(128, true, 1, (unsigned)w1.size(), dps, dw, st));
+  HIP_TRY(hipEventRecord(g.evb, st));
+  HIP_TRY(brams::launch_rodas3_grids(128, false, 1, (unsigned)w128.size(), dps, dw + w1.size(), st));
+  HIP_TRY(hipEventRecord(g.evm, st));
+  const int wps = w64.size() >= (size_t)brams::kRodas3ManyBlocks ? brams::kRodas3ManyWaves : 1;
+  HIP_TRY(brams::launch_rodas3_grids(64, false, wps, (unsigned)w64.size(), dps, dw + w1.size() + w128.size(),
+                                     st));
+  HIP_TRY(hipEventRecord(g.ev1, st));
+  // per-grid counters, then their sums (mask size: the first grid's)
+  brams_chem_stats tot{};
+  long long main_cells = 0;
+  for (int w : w64) main_cells += gss[w >> brams::kGridShift]->sizes[w & brams::kGridBlockMask];
+  g.main_cells = main_cells;
+  g.last_form = -1;
+  for (int i = 0; i < ngrids; ++i) {
+    g.cur = gss[i];
+    gss[i]->blk_hist = true;
+    int rc = collect_stats(st);
+    if (rc < 0) return rc;
+    const brams_chem_stats& s = g.stats;
+    tot.cells += s.cells;
+    tot.blocks += s.blocks;
+    tot.substeps += s.substeps;
+    tot.attempts += s.attempts;
+    tot.failed_blocks += s.failed_blocks;
+    tot.cell_substeps += s.cell_substeps;
+    tot.cell_attempts += s.cell_attempts;
+    tot.forced_blocks += s.forced_blocks;
+    tot.max_forced_err = std::max(tot.max_forced_err, s.max_forced_err);
+    tot.threshold_pivots += s.threshold_pivots;
+    tot.order_threshold_pivots += s.order_threshold_pivots;
+    if (i == 0) tot.mask_nnz = s.mask_nnz;
+    tot.kernel_ms = s.kernel_ms;
+    tot.block1_ms = s.block1_ms;
+    tot.main_ms = s.main_ms;
+    tot.main_cells = s.main_cells;
+  }
+  tot.kernel_form = -1;
+  g.stats = tot;
+  return (int)tot.failed_blocks;
+}
+
 // ------------------------------------------------------------------ surrogate
 int brams_sur_features_raw(const brams_chem_dev_args* a, float* feat, double floor_c) {
   brams::StepParams p;
@@ -814,7 +961,8 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
       if ((rc = ensure_slabs(nslots))) return rc;
       HIP_TRY(brams::launch_rodas3_slab(brams::kSlabList, (unsigned)nslots, q, queue(nullptr, 0, nb), st));
     } else {
-      HIP_TRY(brams::launch_rodas3_list((unsigned)nb, q, st));
+      if (gs.list_wps == 0 && (rc = tune_list(q, nb, st))) return rc;
+      HIP_TRY(brams::launch_rodas3_list((unsigned)nb, q, st, gs.list_wps));
     }
     hipLaunchKernelGGL(brams::cell_history_update, dim3((unsigned)((nfb + 255) / 256)), dim3(256), 0, st, list,
                        nfb, q.last_dt, gs.cell_dt.as<double>());
@@ -843,6 +991,7 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
     s.fallback_cell_attempts = fb_cell_att;
     s.fallback_cell_substeps = fb_cell_sub;
     s.fallback_failed_blocks = fb_failed;
+    s.fallback_wps = gs.list_wps;
     float ms;
     HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); s.features_ms = ms;
     HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[2])); s.mlp_ms = ms;
@@ -1031,7 +1180,7 @@ void brams_chem_finalize(void) {
 
 int brams_chem_set_form(int32_t form) {
   if (form < -1 || form >= kForms) return fail(-2, "kernel form %d out of range (-1..%d)", form, kForms - 1);
-  if (!kSlabEnabled && form >= 0 && form != kFormPerLane)
+  if (!kSlabEnabled && form >= 0 && form < kFormPerLane)
     return fail(-6, "kernel form %d (slab) is switched off in this build", form);
   g.force_form = form;
   return 0;

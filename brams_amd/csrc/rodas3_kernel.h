@@ -96,18 +96,18 @@ __device__ __forceinline__ void block_reduce(double& errmax, int& rej, int& fail
 // 1294 / 120, 2: 1059 / 102, 3: 998 / 98, 4: 939 / 95, 6: 1023 / 112,
 // 8: 1082 / 118, the scratch per lane growing only from 8.0 to 9.2 KB.  A
 // launch of a few thousand blocks has too few waves to use the occupancy
-// and runs faster with the whole register file, so the launch picks WPS by
-// its block count (rodas3_launch.hip).  Crossover measurements (WPS 4 / 1,
-// ms): 6002 blocks (98x98x40) 255 / 285; 2300 blocks (the surrogate's
-// fallback) 44 / 31; 1562 blocks (50x50x40) 93 / 74.
+// and runs faster with the whole register file.  Crossover measurements
+// (WPS 4 / 1, ms): 6002 blocks (98x98x40) 255 / 285; 2300 blocks (the
+// surrogate's fallback) 44 / 31; 1562 blocks (50x50x40) 93 / 74.  The
+// library times WPS 1, 2 and 4 on the first call of each grid and keeps the
+// fastest; launches without a timed choice pick by block count.
 constexpr int kRodas3ManyWaves = 4;        // WPS of large launches
 constexpr int kRodas3ManyBlocks = 4096;    // from this many blocks on
 
 // LIST: the blocks are consecutive WG-sized runs of p.cell_list (fallback of
 // the surrogate step) instead of the AllocIndex partition.
-template <int WG, bool DYN, bool LIST, int WPS>
-__global__ __launch_bounds__(WG, WPS) void rodas3_blocks(StepParams p, int block0) {
-  const int b = (!LIST && p.block_order) ? p.block_order[blockIdx.x] : block0 + blockIdx.x;
+template <int WG, bool DYN, bool LIST>
+__device__ __forceinline__ void integrate_block(const StepParams& p, int b) {
   int start, ncell;
   if constexpr (LIST) {                             // fallback blocks of listed cells
     start = b * WG;
@@ -341,6 +341,26 @@ __global__ __launch_bounds__(WG, WPS) void rodas3_blocks(StepParams p, int block
       for (int w = 0; w < MECH_MASK_WORDS; ++w) p.mask[w] = s_mask[w];
     }
   }
+}
+
+template <int WG, bool DYN, bool LIST, int WPS>
+__global__ __launch_bounds__(WG, WPS) void rodas3_blocks(StepParams p, int block0) {
+  const int b = (!LIST && p.block_order) ? p.block_order[blockIdx.x] : block0 + blockIdx.x;
+  integrate_block<WG, DYN, LIST>(p, b);
+}
+
+// Several grids in one launch (the nested grids of one model step, which the
+// reference integrates one call per grid, chemistry.f90:313-320): workgroup
+// w integrates block (work[w] & kGridBlockMask) of the grid whose parameters
+// are grids[work[w] >> kGridShift].  Blocks never exchange data, so a block
+// gives the same result here as in its own grid's launch.
+constexpr int kGridShift = 24;
+constexpr int kGridBlockMask = (1 << kGridShift) - 1;
+template <int WG, bool DYN, int WPS>
+__global__ __launch_bounds__(WG, WPS) void rodas3_grids(const StepParams* __restrict__ grids,
+                                                        const int* __restrict__ work) {
+  const int w = work[blockIdx.x];
+  integrate_block<WG, DYN, false>(grids[w >> kGridShift], w & kGridBlockMask);
 }
 
 }  // namespace brams

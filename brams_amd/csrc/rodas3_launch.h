@@ -1,4 +1,6 @@
-// Host entry of the RODAS3 block kernels (defined in rodas3_launch.hip).
+// Host entry of the RODAS3 block kernels (defined in rodas3_launch.hip,
+// rodas3_launch_b.hip and rodas3_grids.hip, split so the units compile in
+// parallel).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -8,9 +10,21 @@ struct StepParams;
 
 // nblocks workgroups of wg (64 or 128) lanes over blocks block0.. of p;
 // dyn: block-1 form (mask recomputed on every attempt and published).
+// wps: waves per SIMD of the 64-lane form (1, 2 or 4; 0: by block count).
 hipError_t launch_rodas3(int wg, bool dyn, unsigned nblocks, const StepParams& p, int block0,
-                         hipStream_t st);
-// nblocks workgroups of 64 lanes over consecutive 64-entry runs of p.cell_list.
-hipError_t launch_rodas3_list(unsigned nblocks, const StepParams& p, hipStream_t st);
+                         hipStream_t st, int wps = 0);
+// nblocks workgroups of 64 lanes over consecutive 64-entry runs of p.cell_list
+// (wps 1 or 2).
+hipError_t launch_rodas3_list(unsigned nblocks, const StepParams& p, hipStream_t st, int wps = 1);
+
+// the parts of the above that live in rodas3_launch_b.hip
+hipError_t launch_rodas3_128(bool dyn, unsigned nblocks, const StepParams& p, int block0, hipStream_t st);
+hipError_t launch_rodas3_64w2(unsigned nblocks, const StepParams& p, int block0, hipStream_t st);
+
+// Several grids in one launch (rodas3_grids): nwork workgroups of wg lanes,
+// workgroup w on block work[w] & kGridBlockMask of grid work[w] >> kGridShift,
+// whose parameters are grids[] (device memory).  wps 1 or 4 (64 lanes only).
+hipError_t launch_rodas3_grids(int wg, bool dyn, int wps, unsigned nwork, const StepParams* grids,
+                               const int* work, hipStream_t st);
 
 }  // namespace brams

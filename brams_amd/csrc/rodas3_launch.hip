@@ -1,11 +1,11 @@
-// Launch wrappers of the RODAS3 block kernels (rodas3_kernel.h).
+// Launch wrappers of the RODAS3 block kernels (rodas3_kernel.h), 64-lane
+// partition forms.
 //
-// The kernels are instantiated in this translation unit only, so the rest of
-// the host library compiles without them; wg is the workgroup size (64 or
-// 128 cells per block), dyn selects the block-1 form that recomputes and
-// publishes the sparsity mask.  Large partition launches take the
-// kRodas3ManyWaves register budget, every other launch the whole register
-// file (rodas3_kernel.h).
+// The kernels are instantiated in the rodas3_launch*.hip and rodas3_grids.hip
+// units only, so the rest of the host library compiles without them; wg is
+// the workgroup size (64 or 128 cells per block), dyn selects the block-1
+// form that recomputes and publishes the sparsity mask, wps the register
+// budget (waves per SIMD) of the 64-lane form.
 #include <hip/hip_runtime.h>
 
 #include "rodas3_kernel.h"
@@ -14,28 +14,22 @@
 namespace brams {
 
 hipError_t launch_rodas3(int wg, bool dyn, unsigned nblocks, const StepParams& p, int block0,
-                         hipStream_t st) {
+                         hipStream_t st, int wps) {
   if (nblocks == 0) return hipSuccess;
-  if (wg == 64 && dyn)
+  if (wg == 128) return launch_rodas3_128(dyn, nblocks, p, block0, st);
+  if (wg != 64) return hipErrorInvalidValue;
+  if (dyn) {
     hipLaunchKernelGGL((rodas3_blocks<64, true, false, 1>), dim3(nblocks), dim3(64), 0, st, p, block0);
-  else if (wg == 64 && nblocks >= (unsigned)kRodas3ManyBlocks)
-    hipLaunchKernelGGL((rodas3_blocks<64, false, false, kRodas3ManyWaves>), dim3(nblocks), dim3(64), 0, st, p,
-                       block0);
-  else if (wg == 64)
+    return hipGetLastError();
+  }
+  if (wps == 0) wps = nblocks >= (unsigned)kRodas3ManyBlocks ? kRodas3ManyWaves : 1;
+  if (wps == 2) return launch_rodas3_64w2(nblocks, p, block0, st);
+  if (wps == 4)
+    hipLaunchKernelGGL((rodas3_blocks<64, false, false, 4>), dim3(nblocks), dim3(64), 0, st, p, block0);
+  else if (wps == 1)
     hipLaunchKernelGGL((rodas3_blocks<64, false, false, 1>), dim3(nblocks), dim3(64), 0, st, p, block0);
-  else if (wg == 128 && dyn)
-    hipLaunchKernelGGL((rodas3_blocks<128, true, false, 1>), dim3(nblocks), dim3(128), 0, st, p, block0);
-  else if (wg == 128)
-    hipLaunchKernelGGL((rodas3_blocks<128, false, false, 1>), dim3(nblocks), dim3(128), 0, st, p, block0);
   else
     return hipErrorInvalidValue;
-  return hipGetLastError();
-}
-
-hipError_t launch_rodas3_list(unsigned nblocks, const StepParams& p, hipStream_t st) {
-  if (nblocks == 0) return hipSuccess;
-  if (!p.cell_list) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((rodas3_blocks<64, false, true, 1>), dim3(nblocks), dim3(64), 0, st, p, 0);
   return hipGetLastError();
 }
 

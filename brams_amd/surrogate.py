@@ -67,7 +67,8 @@ class SurStats(C.Structure):
                 ("fallback_block_attempts", C.c_int64), ("fallback_cell_attempts", C.c_int64),
                 ("fallback_cell_substeps", C.c_int64), ("features_ms", C.c_float),
                 ("mlp_ms", C.c_float), ("apply_ms", C.c_float), ("fallback_ms", C.c_float),
-                ("total_ms", C.c_float), ("fallback_failed_blocks", C.c_int64)]
+                ("total_ms", C.c_float), ("fallback_failed_blocks", C.c_int64),
+                ("fallback_wps", C.c_int32)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reasons"}

@@ -119,8 +119,8 @@ typedef struct brams_chem_stats {
                                     reference orders on every block-1 attempt
                                     (chem_spack_rodas3_dyndt.f90:470-506): 0 means Sparse's
                                     threshold test accepts every pivot of the fixed order */
-    int32_t kernel_form;         /* kernel form that integrated the 64-cell blocks: 0..2 the
-                                    slab layouts, 3 the per-lane kernel */
+    int32_t kernel_form;         /* kernel form that integrated the 64-cell blocks (see
+                                    brams_chem_set_form; -1: none, or several grids) */
 } brams_chem_stats;
 
 /* Select the HIP device and load the mechanism tables.  Idempotent. */
@@ -155,6 +155,22 @@ int brams_chem_rodas3_dyndt_device(const brams_chem_dev_args *args);
 int brams_chem_rodas3_dyndt_device_noalloc(const brams_chem_dev_args *args);
 
 /*
+ * The nested grids of one model step in one pass.  chemistry_driver calls
+ * chem_rodas3_dyndt once per grid (chemistry.f90:313-320); this integrates
+ * grids[0..ngrids) together: block 1 of every grid first (each publishes its
+ * own grid's mask), then every other block of every grid in shared
+ * launches, the 64-cell blocks longest first across the grids from the
+ * second call on.  Each grid keeps its own partition, mask,
+ * last_accepted_dt and counters, so the results equal those of ngrids
+ * brams_chem_rodas3_dyndt_device calls.  1 <= ngrids <= 16, grids of
+ * distinct extents, one stream (all grids' stream fields equal).
+ * brams_chem_get_stats then holds the sums over the grids (mask_nnz: the
+ * first grid's) and brams_chem_block_status the last grid's blocks.
+ * Synchronises the stream; returns the total number of failed blocks.
+ */
+int brams_chem_rodas3_dyndt_device_grids(const brams_chem_dev_args *grids, int32_t ngrids);
+
+/*
  * Block partition of one grid, AllocIndex (chem_spack_utils.f90:39-183):
  * interior cells in j->i->k order cut into floor(N/block_size) blocks, the
  * remainder spread +1 over the first blocks.  Writes block_end[0..nob-1] when
@@ -178,11 +194,13 @@ int brams_chem_block_thresholds(int32_t *counts, int32_t max_nob);
 
 /*
  * Kernel form of the 64-cell blocks (no reference counterpart).  The forms
- * give bit-identical results and differ in speed: 0..2 keep the per-cell
- * working set in registers, LDS (32, 96 or 0 LU slots) and a per-slot
- * scratch slab, run as persistent workgroups over a device work queue; 3 is
- * the per-lane kernel.  -1 (default): the first call of each grid times every
- * form on a scratch copy of its state and keeps the fastest for the grid.
+ * give bit-identical results and differ in speed: 3, 4 and 5 are the
+ * per-lane kernel with the register budget of 1, 2 and 4 waves per SIMD;
+ * 0..2 (the slab layouts: per-cell working set in registers, LDS and a
+ * per-slot scratch slab, persistent workgroups over a device work queue) are
+ * switched off in this build and refused (-6).  -1 (default): the first call
+ * of each grid times every available form on a scratch copy of its state and
+ * keeps the fastest for the grid.
  */
 int brams_chem_set_form(int32_t form);
 /* The current grid's chosen form and the autotuner's timings (ms, one per

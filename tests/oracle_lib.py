@@ -295,6 +295,16 @@ def block_thresholds(nob):
     return c
 
 
+def block_forced(nob):
+    """Per-block largest error accepted at dt_min of the last oracle step
+    (float; inf when it overflowed)."""
+    c = np.zeros(nob, np.float32)
+    lib = oracle()
+    lib.ora_block_forced.argtypes = [C.POINTER(C.c_float), C.c_int]
+    lib.ora_block_forced(c.ctypes.data_as(C.POINTER(C.c_float)), nob)
+    return c
+
+
 def block_failed(nob):
     """Per-block failure flags of the last oracle step."""
     c = np.zeros(nob, np.int32)

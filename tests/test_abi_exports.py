@@ -74,14 +74,26 @@ def test_surrogate_step_rejects_parallel_split():
 
 def test_slab_forms_are_refused_while_switched_off():
     """The slab kernel forms (0..2) are switched off in this build: the
-    per-lane form (3) and autotune (-1, which then keeps the per-lane form)
-    are accepted, a slab form is refused with an error and no device work."""
+    per-lane forms (3, 4, 5: 1, 2, 4 waves per SIMD) and autotune (-1) are
+    accepted, a slab form is refused with an error and no device work."""
     L = abi.lib()
     assert L.brams_chem_set_form(0) == -6
     assert b"switched off" in L.brams_chem_last_error()
-    assert L.brams_chem_set_form(7) == -2
-    assert L.brams_chem_set_form(3) == 0
+    assert L.brams_chem_set_form(6) == -2
+    for form in (3, 4, 5):
+        assert L.brams_chem_set_form(form) == 0
     assert L.brams_chem_set_form(-1) == 0
+
+
+def test_grids_entry_checks_its_arguments():
+    """brams_chem_rodas3_dyndt_device_grids refuses a bad grid count before
+    any device work."""
+    L = abi.lib()
+    L.brams_chem_rodas3_dyndt_device_grids.argtypes = [C.POINTER(abi.ChemDevArgs), C.c_int32]
+    arr = (abi.ChemDevArgs * 17)()
+    assert L.brams_chem_rodas3_dyndt_device_grids(arr, 0) == -2
+    assert L.brams_chem_rodas3_dyndt_device_grids(arr, 17) == -2
+    assert b"ngrids" in L.brams_chem_last_error()
 
 
 def test_noalloc_entry_refuses_grids_beyond_its_limit():

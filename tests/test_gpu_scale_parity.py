@@ -11,10 +11,16 @@ and accepted sub-steps equal, last_accepted_dt within rel 1e-9 (the
 controller's err**(-1/3) carries the round-off of fused multiply-adds, about
 1e-11 relative), threshold-rule counts equal; globally the mask size.
 
-A block fails where the reference would stop the model (a zero pivot or a
-non-finite error sum; chem_spack_rodas3_dyndt.f90 aborts there): every block
-the GPU flags is added to the oracle's list, the two sets of failed blocks
-must be equal, and failed blocks are left out of the value checks.
+Some blocks of the plume blow up: they accept steps at dt_min whose error
+overflows (above 3.4e38), and some of them end in a non-finite state or a
+zero pivot, where the reference stops the model
+(chem_spack_rodas3_dyndt.f90 aborts in Sparse1.3).  Which of the blown-up
+blocks ends that way depends on round-off (a different FMA contraction, a
+last-bit difference of exp/pow), so the check there is on the set of
+blown-up blocks (failed, or a forced error of 1e30 or more): every
+block the GPU flags is added to the oracle's list, the two sets must be
+equal on the blocks run, both steps must report a failure when either does,
+and blown-up blocks are left out of the value checks.
 
 Both orientations of the split day/night field are run, because cell 1
 (k=2, i=ia, j=ja) sets the mask for the whole call: "split" puts it in
@@ -30,6 +36,7 @@ from brams_amd.workload import Grid, make_workload, tolerances
 pytestmark = pytest.mark.gpu
 
 MECH = load_mechanism()
+BLOWN = 1e30      # a forced acceptance at least this large marks a blown-up block
 
 
 def sample_blocks(nob, every, head=0):
@@ -79,27 +86,30 @@ def run_case(grid, sza, tol, every, head=0, seed=0):
     st = abi.stats()
     gpu = ds.species_host()
     gdt = ds.last_dt.cpu().numpy()
-    gatt, gsub, gfail, _ = abi.block_status(nob)
+    gatt, gsub, gfail, gforced = abi.block_status(nob)
     gth = abi.block_thresholds(nob)
     del ds
-    gbad = np.nonzero(gfail)[0]
+    gblown = (gfail != 0) | (gforced >= BLOWN)
+    gbad = np.nonzero(gblown)[0]
     blocks = np.union1d(sample_blocks(nob, every, head), gbad[gbad > 0]).astype(np.int32)
     hs, ost, mask, orc = oracle_lib.run_oracle_blocks(wl, atol, rtol, blocks)
     oatt, osub = oracle_lib.block_counts(nob)
     oth = oracle_lib.block_thresholds(nob)
     ofail = oracle_lib.block_failed(nob)
+    oblown = (ofail != 0) | (oracle_lib.block_forced(nob) >= BLOWN)
     ran = np.concatenate([[0], blocks])
-    ids = ran[(gfail[ran] == 0) & (ofail[ran] == 0)]
+    ids = ran[~gblown[ran] & ~oblown[ran]]
     worst, nfail = compare(grid, wl, gpu, hs.sc_p, ids)
     flips = int(np.sum(gatt[ids] != oatt[ids]))
     print("  %d blocks, %d run on the oracle | gpu %.1f ms attempts/cell %.1f | mask %d/%d | "
-          "failed gpu %s oracle %s | attempt mismatches %d | worst %.3g nfail %d | threshold %d"
+          "blown up gpu %s (failed %s) oracle %s (failed %s) | attempt mismatches %d | worst %.3g "
+          "nfail %d | threshold %d"
           % (nob, len(ran), st.kernel_ms, st.cell_attempts / st.cells, st.mask_nnz, ost.mask_nnz,
-             gbad.tolist(), np.nonzero(ofail)[0].tolist(), flips, worst, nfail, int(gth.sum())),
-          flush=True)
-    assert np.array_equal(gfail[ran] != 0, ofail[ran] != 0)
-    assert rc == orc == len(gbad)
-    assert gfail[0] == 0
+             gbad.tolist(), np.nonzero(gfail)[0].tolist(), ran[oblown[ran]].tolist(),
+             np.nonzero(ofail)[0].tolist(), flips, worst, nfail, int(gth.sum())), flush=True)
+    assert np.array_equal(gblown[ran], oblown[ran])
+    assert (rc > 0) == (orc > 0)
+    assert rc == int(np.count_nonzero(gfail)) and not gblown[0]
     assert st.mask_nnz == ost.mask_nnz
     assert flips == 0
     assert np.array_equal(gsub[ids], osub[ids])
@@ -113,8 +123,8 @@ def run_case(grid, sza, tol, every, head=0, seed=0):
 @pytest.mark.parametrize("tol", ["baseline", "reference"])
 @pytest.mark.parametrize("sza", ["split", "split_night_first"])
 def test_200x200x40_matches_oracle(sza, tol):
-    """The plain 1.6M-cell step through the >= 4096-block kernel form
-    (four waves per SIMD); 501 sampled blocks plus block 1."""
+    """The plain 1.6M-cell step (25 000 blocks, the autotuned per-lane
+    form); 501 sampled blocks plus block 1."""
     st = run_case(Grid(200, 200, 40, dtlt=120.0), sza, tol, every=50)
     assert st.blocks == 25000
 
@@ -126,6 +136,5 @@ def test_200x200x40_matches_oracle(sza, tol):
 def test_nested_grids_full_depth_match_oracle(nx, dt, sza, tol):
     """The nested grids of the three-grid workload at full depth (40
     levels): their first 32 blocks have 65 cells (128-lane workgroups) and
-    are all sampled; 98x98x40 (6002 blocks) runs the large-launch kernel
-    form, 50x50x40 (1562 blocks) the small-launch one."""
+    are all sampled; 98x98x40 has 6002 blocks, 50x50x40 1562."""
     run_case(Grid(nx, nx, 40, dtlt=dt), sza, tol, every=25, head=40)
