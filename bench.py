@@ -66,22 +66,34 @@ MLP_FLOP_PER_CELL = 3 * 2 * (68 * 256 + 3 * 256 * 256 + 256 * 47)
 BF16_PEAK_TFLOPS = 2500.0
 # HBM traffic measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by
 # the counters' reading of a 1 GiB device copy: x2.0 and x1.0) on this
-# workload, profiles/rodas3_hbm_traffic_pmc.md.  Per cell-attempt, the same
-# in every launch measured (the spill traffic of the per-lane LU): the
-# partition kernel (four waves per SIMD) 48.9 KB read + 14.4 KB written, the
-# fallback list kernel (one wave per SIMD) 29.3 KB + 8.7 KB; the fused MLP
-# 2.90 GB per launch over 1.6M cells (1815 B per cell).
-RODAS3_BYTES_PER_CELL_ATTEMPT = 48900 + 14400
-RODAS3_LIST_BYTES_PER_CELL_ATTEMPT = 29300 + 8700
-MLP_BYTES_PER_CELL = 1815
+# workload, profiles/rodas3_hbm_traffic_pmc.md.  Per cell-attempt (read +
+# written bytes), the same in every launch of a kernel: the spill traffic of
+# the per-lane LU and stage vectors, which depends on the register budget
+# (waves per SIMD).  The fused MLP: bytes per cell of one launch.  A kernel
+# without a measurement reports traffic null.
+RODAS3_BYTES_PER_CELL_ATTEMPT = {
+    "rodas3_blocks<64,false,false,4>": 50900 + 14000,
+    "rodas3_blocks<64,false,false,2>": 41000 + 10900,
+    "rodas3_blocks<64,false,false,1>": 31600 + 8600,
+    "rodas3_blocks<64,false,true,2>": 40700 + 11100,
+    "rodas3_blocks<64,false,true,1>": 31400 + 8600,
+}
+MLP_BYTES_PER_CELL = 1825
+FORM_WPS = {3: 1, 4: 2, 5: 4}
 
 
-def cpu_baseline(warm_wl, cold_wl, atol, rtol, warm_every=4, cold_every=40):
+def spill_traffic(kernel, cell_attempts):
+    b = RODAS3_BYTES_PER_CELL_ATTEMPT.get(kernel)
+    return None if b is None else b * cell_attempts
+
+
+def cpu_baseline(warm_wl, cold_wl, atol, rtol, warm_every=1, cold_every=4):
     """The oracle (C restatement of the reference step, gcc -O2, pthreads over
-    blocks) on every warm_every-th block of the warm state, with each block's
-    carried last_accepted_dt (the reference's semantics for those blocks),
-    and on every cold_every-th block of the seeded cold state; block 1 runs
-    first in both, as it fixes the mask."""
+    blocks) on every warm_every-th block of the warm state (every block: the
+    whole warm step), with each block's carried last_accepted_dt (the
+    reference's semantics for those blocks), and on every cold_every-th block
+    of the seeded cold state; block 1 runs first in both, as it fixes the
+    mask."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import oracle_lib
@@ -89,7 +101,7 @@ def cpu_baseline(warm_wl, cold_wl, atol, rtol, warm_every=4, cold_every=40):
     out = {}
     for tag, wl, every in (("warm", warm_wl, warm_every), ("cold", cold_wl, cold_every)):
         nob = len(wl["block_end"])
-        blocks = np.arange(every, nob, every, dtype=np.int32)
+        blocks = np.arange(max(every, 1), nob, every, dtype=np.int32)
         cells = int(wl["block_end"][0]) + int(np.sum(np.asarray(wl["block_end"])[blocks]))
         t0 = time.perf_counter()
         hs, st, mask, rc = oracle_lib.run_oracle_blocks(wl, atol, rtol, blocks, nthreads=cores)
@@ -101,13 +113,17 @@ def cpu_baseline(warm_wl, cold_wl, atol, rtol, warm_every=4, cold_every=40):
     return {"value": w["value"], "unit": "cell-steps/s", "cores": cores, "kind": "port",
             "calibration": calib,
             "sample": "oracle (C restatement of chem_rodas3_dyndt with the reference's kernels' "
-                      "arithmetic, gcc -O2 -ffp-contract=off, pthreads over blocks) on block 1 and every "
-                      "%dth block of the warm state (%d blocks, %d cells x 1 step, carried last_accepted_dt, "
-                      "%.1f s wall, %d cell-attempts)" % (warm_every, w["blocks"], w["cells"], w["wall_s"],
-                                                          w["cell_attempts"]),
+                      "arithmetic, gcc -O2 -ffp-contract=off, pthreads over blocks) on %s of the warm state "
+                      "(%d blocks, %d cells x 1 step, carried last_accepted_dt, %.1f s wall, "
+                      "%d cell-attempts)" % (_which(warm_every), w["blocks"], w["cells"], w["wall_s"],
+                                             w["cell_attempts"]),
             "cold_value": c["value"],
-            "cold_sample": "block 1 and every %dth block of the seeded cold state (%d cells, %.1f s wall, "
-                           "%d cell-attempts)" % (cold_every, c["cells"], c["wall_s"], c["cell_attempts"])}
+            "cold_sample": "%s of the seeded cold state (%d cells, %.1f s wall, %d cell-attempts)"
+                           % (_which(cold_every), c["cells"], c["wall_s"], c["cell_attempts"])}
+
+
+def _which(every):
+    return "every block" if every <= 1 else "block 1 and every %dth block" % every
 
 
 def calibrate_oracle(wl, atol, rtol, cores, every=200):
@@ -291,10 +307,15 @@ def main():
         torch.cuda.synchronize()
         st = abi.stats()
         if t == 0:
-            cold = {"ms": 1e3 * (time.perf_counter() - t0), "kernel_ms": st.kernel_ms,
+            # the first call of a grid also times the kernel forms (autotuner):
+            # the cold step's rate is taken over its own kernels' event time
+            form, tune_ms = abi.tuning()
+            cold = {"kernel_ms": st.kernel_ms, "call_ms_with_autotune": 1e3 * (time.perf_counter() - t0),
                     "attempts_per_cell": st.cell_attempts / st.cells,
                     "substeps_per_cell": st.cell_substeps / st.cells,
-                    "cell_steps_per_s": grid.ncell / (time.perf_counter() - t0)}
+                    "cell_steps_per_s": grid.ncell / (st.kernel_ms * 1e-3),
+                    "autotune": {"form": form, "kernel": "rodas3_blocks<64,false,false,%d>" % FORM_WPS.get(form, 0),
+                                 "ms_per_form": {"wps%d" % FORM_WPS[f]: float(tune_ms[f]) for f in FORM_WPS}}}
         valid &= trusted_cells(wl["block_end"], dev) & physical_cells(ref, fbuf)
         log("spin-up step %d: %.1f ms (%d cell-attempts)" % (t, st.kernel_ms, st.cell_attempts))
         if t == 0:
@@ -364,9 +385,10 @@ def main():
             block_spread("last fp64 step")
     fp64_value = grid.ncell * K / wall_b
 
-    plain = {"kernel": "rodas3_blocks<64,false,false,4>", "bound": "fp64", "unit": "TFLOP/s",
+    pk = "rodas3_blocks<64,false,false,%d>" % FORM_WPS.get(abi.stats().kernel_form, 0)
+    plain = {"kernel": pk, "bound": "fp64", "unit": "TFLOP/s",
              "avg_ms": b_main_ms / K, "achieved": b_flops_main / (b_main_ms * 1e-3) / 1e12,
-             "peak": FP64_PEAK_TFLOPS, "traffic": RODAS3_BYTES_PER_CELL_ATTEMPT * b_att_main / K,
+             "peak": FP64_PEAK_TFLOPS, "traffic": spill_traffic(pk, b_att_main / K),
              "algorithmic_bytes": 528 * grid.ncell}
     plain["frac"] = plain["achieved"] / plain["peak"]
     if sur is None:
@@ -380,9 +402,10 @@ def main():
         fb_flops = sum(rodas3_flops(s.fallback_cells, s.fallback_cell_substeps, s.fallback_cell_attempts)
                        for s in sstats)
         fb_att = sum(s.fallback_cell_attempts for s in sstats)
-        fallback = {"kernel": "rodas3_blocks<64,false,true,1>", "bound": "fp64", "unit": "TFLOP/s",
+        fk = "rodas3_blocks<64,false,true,%d>" % sstats[-1].fallback_wps
+        fallback = {"kernel": fk, "bound": "fp64", "unit": "TFLOP/s",
                     "avg_ms": fb_ms / K, "achieved": fb_flops / max(fb_ms * 1e-3, 1e-9) / 1e12,
-                    "peak": FP64_PEAK_TFLOPS, "traffic": RODAS3_LIST_BYTES_PER_CELL_ATTEMPT * fb_att / K,
+                    "peak": FP64_PEAK_TFLOPS, "traffic": spill_traffic(fk, fb_att / K),
                     "algorithmic_bytes": 528 * sum(s.fallback_cells for s in sstats) / K}
         fallback["frac"] = fallback["achieved"] / fallback["peak"]
         mlp = {"kernel": "mlp_infer<256,4,8,2>", "bound": "mfma", "unit": "TFLOP/s",
@@ -425,7 +448,8 @@ def main():
                 "unit": dominant["unit"], "frac": dominant["frac"], "traffic": dominant["traffic"],
                 "algorithmic_bytes": dominant["algorithmic_bytes"],
                 "traffic_note": "HBM bytes per launch: PMC-measured bytes per cell-attempt (RODAS3) or per "
-                                "cell (MLP) x this run's counts, profiles/rodas3_hbm_traffic_pmc.md",
+                                "cell (MLP) x this run's counts, profiles/rodas3_hbm_traffic_pmc.md; "
+                                "null where that kernel form was not measured",
                 "kernel": dominant["kernel"], "avg_ms": dominant["avg_ms"],
                 "peak_note": "FP64 vector for the RODAS3 kernels (no MFMA issued; MI355X FP64 vector "
                              "and matrix peaks are both 78.6 TFLOP/s), dense BF16 MFMA for mlp_infer",

@@ -76,8 +76,14 @@ struct DevBuf {
 constexpr int kFormPerLane = brams::kSlabVariants;      // first per-lane form
 constexpr int kPerLaneWps[3] = {1, 2, 4};
 constexpr int kForms = brams::kSlabVariants + 3;
-// list (fallback) launches: per-lane kernel at 1 or 2 waves per SIMD
+// List (fallback) launches: the per-lane kernel at 1 or 2 waves per SIMD,
+// by the launch's block count.  Measured on the 1.6M-cell rollout (ms per
+// launch, 1 / 2 waves per SIMD): 19 400 blocks 246 / 190; the 2000..6000
+// blocks of the warm steps 35.5 / 39.6 on average over 24 steps.  Timing
+// each call's choice online was too noisy to pick reliably (consecutive
+// calls differ by 10%), so the crossover is fixed.
 constexpr int kListWps[2] = {1, 2};
+constexpr int kListManyBlocks = 8192;
 // The slab forms are switched off: on gfx950 their kernels did not
 // terminate, even for a one-block grid with the attempts capped at 300,
 // while a host build of the same block integration matches the oracle bit
@@ -101,8 +107,6 @@ struct GridState {
   std::vector<float> h_forced;
   int form = -1;                 // kernel form of the 64-cell blocks (-1: not tuned yet)
   float tune_ms[kForms] = {};
-  int list_wps = 0;              // waves per SIMD of the fallback launches (0: not tuned yet)
-  float list_tune_ms[2] = {};
   bool same(int a1, int a2, int a3, int b1, int b2, int b3, int b4) const {
     return m1 == a1 && m2 == a2 && m3 == a3 && ia == b1 && iz == b2 && ja == b3 && jz == b4;
   }
@@ -378,35 +382,6 @@ int autotune(brams::StepParams p, int first64, int nob, hipStream_t st) {
     if (ms < gs.tune_ms[best]) best = f;
   }
   gs.form = best;
-  return 0;
-}
-
-// The same for the surrogate's fallback launches: on the first fallback of
-// a grid both list forms (1 and 2 waves per SIMD) integrate the call's
-// list on a scratch copy of the state and step sizes; the faster is kept for
-// the grid.  q is the real launch's parameters (sc_p, last_dt of the list).
-int tune_list(const brams::StepParams& q, int nb, hipStream_t st) {
-  GridState& gs = *g.cur;
-  const size_t nsc = sizeof(float) * MECH_NS * (size_t)q.n3d;
-  HIP_TRY(g.tune_sc.ensure(nsc));
-  HIP_TRY(g.tune_dt.ensure(sizeof(double) * nb));
-  brams::StepParams t = q;
-  t.sc_p = g.tune_sc.as<float>();
-  t.last_dt = g.tune_dt.as<double>();
-  int best = 0;
-  for (int k = 0; k < 2; ++k) {
-    HIP_TRY(hipMemcpyAsync(t.sc_p, q.sc_p, nsc, hipMemcpyDeviceToDevice, st));
-    HIP_TRY(hipMemcpyAsync(t.last_dt, q.last_dt, sizeof(double) * nb, hipMemcpyDeviceToDevice, st));
-    HIP_TRY(hipEventRecord(g.ev0, st));
-    HIP_TRY(brams::launch_rodas3_list((unsigned)nb, t, st, kListWps[k]));
-    HIP_TRY(hipEventRecord(g.ev1, st));
-    HIP_TRY(hipEventSynchronize(g.ev1));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, g.ev0, g.ev1));
-    gs.list_tune_ms[k] = ms;
-    if (ms < gs.list_tune_ms[best]) best = k;
-  }
-  gs.list_wps = kListWps[best];
   return 0;
 }
 
@@ -932,6 +907,7 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
   //    1.6M-cell rollout (68 against 57 ms): scattered gathers cost more than
   //    the more uniform blocks save.
   long long fb_att = 0, fb_cell_att = 0, fb_cell_sub = 0, fb_failed = 0;
+  int fb_form = -1;
   if (nfb > 0) {
     const int nb = (nfb + 63) / 64;
     int* list = g.sur_list.as<int>();
@@ -961,8 +937,8 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
       if ((rc = ensure_slabs(nslots))) return rc;
       HIP_TRY(brams::launch_rodas3_slab(brams::kSlabList, (unsigned)nslots, q, queue(nullptr, 0, nb), st));
     } else {
-      if (gs.list_wps == 0 && (rc = tune_list(q, nb, st))) return rc;
-      HIP_TRY(brams::launch_rodas3_list((unsigned)nb, q, st, gs.list_wps));
+      fb_form = nb >= kListManyBlocks ? 1 : 0;
+      HIP_TRY(brams::launch_rodas3_list((unsigned)nb, q, st, kListWps[fb_form]));
     }
     hipLaunchKernelGGL(brams::cell_history_update, dim3((unsigned)((nfb + 255) / 256)), dim3(256), 0, st, list,
                        nfb, q.last_dt, gs.cell_dt.as<double>());
@@ -991,7 +967,7 @@ int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams
     s.fallback_cell_attempts = fb_cell_att;
     s.fallback_cell_substeps = fb_cell_sub;
     s.fallback_failed_blocks = fb_failed;
-    s.fallback_wps = gs.list_wps;
+    s.fallback_wps = fb_form < 0 ? 0 : kListWps[fb_form];
     float ms;
     HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); s.features_ms = ms;
     HIP_TRY(hipEventElapsedTime(&ms, ev[1], ev[2])); s.mlp_ms = ms;

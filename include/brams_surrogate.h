@@ -100,8 +100,9 @@ typedef struct brams_sur_stats {
     float features_ms, mlp_ms, apply_ms, fallback_ms, total_ms;
     int64_t fallback_failed_blocks;  /* fallback blocks ended by a zero pivot, a non-finite
                                         state or the attempt bound (also the return value) */
-    int32_t fallback_wps;            /* waves per SIMD of the fallback kernel, timed on the
-                                        grid's first fallback (0: no fallback yet) */
+    int32_t fallback_wps;            /* waves per SIMD of this call's fallback kernel (0: no
+                                        fallback): 2 from 8192 fallback blocks on, else 1; the
+                                        forms give bit-identical results */
 } brams_sur_stats;
 
 /* Raw features [ncell][BRAMS_SUR_NIN] f32 of the current state (dataset). */
