@@ -72,11 +72,11 @@ BF16_PEAK_TFLOPS = 2500.0
 # (waves per SIMD).  The fused MLP: bytes per cell of one launch.  A kernel
 # without a measurement reports traffic null.
 RODAS3_BYTES_PER_CELL_ATTEMPT = {
-    "rodas3_blocks<64,false,false,4>": 50900 + 14000,
-    "rodas3_blocks<64,false,false,2>": 41000 + 10900,
-    "rodas3_blocks<64,false,false,1>": 31600 + 8600,
-    "rodas3_blocks<64,false,true,2>": 40700 + 11100,
-    "rodas3_blocks<64,false,true,1>": 31400 + 8600,
+    "rodas3_blocks<64,false,false,4>": 49000 + 14200,
+    "rodas3_blocks<64,false,false,2>": 39200 + 10800,
+    "rodas3_blocks<64,false,false,1>": 29700 + 8300,
+    "rodas3_blocks<64,false,true,2>": 39000 + 11100,
+    "rodas3_blocks<64,false,true,1>": 29300 + 8500,
 }
 MLP_BYTES_PER_CELL = 1825
 FORM_WPS = {3: 1, 4: 2, 5: 4}

@@ -254,11 +254,12 @@ def main():
     w("")
     # ---------------- factor
     w("// spFactor.c:336-401 on the fixed pivot order; lu[] keeps L (reciprocal")
-    w("// diagonal) and unit-diagonal U.  Returns nonzero on a zero pivot.  *viol")
-    w("// is increased by the pivots that fail Sparse1.3's threshold rule, which")
-    w("// accepts a pivot only if RelThreshold (1e-3, spAllocate.c:191) times the")
-    w("// largest other entry of its reduced column is below |pivot|")
+    w("// diagonal) and unit-diagonal U.  Returns nonzero on a zero pivot.  With")
+    w("// COUNT, *viol is increased by the pivots that fail Sparse1.3's threshold")
+    w("// rule, which accepts a pivot only if RelThreshold (1e-3, spAllocate.c:191)")
+    w("// times the largest other entry of its reduced column is below |pivot|")
     w("// (spFactor.c:202-203, FindBiggestInColExclude spFactor.c:2019-2050).")
+    w("template <bool COUNT>")
     w("__device__ __forceinline__ int mech_factor(const double* rk, const double* y, double igdt,")
     w("                                           const unsigned* mw, double* lu, int* viol) {")
     w("  int bad = 0, nv = 0;")
@@ -282,15 +283,15 @@ def main():
         for I in below:
             w("    lu[%d] = d%d;" % (slot[(I, J)], I))
         if below:
-            w("    { double cm = fabs(d%d);" % below[0])
+            w("    if constexpr (COUNT) { double cm = fabs(d%d);" % below[0])
             for I in below[1:]:
                 w("      cm = fmax(cm, fabs(d%d));" % I)
             w("      nv += (cm * 1.0e-3 >= fabs(d%d)) ? 1 : 0; }" % J)
         else:
-            w("    nv += (d%d == 0.0) ? 1 : 0;" % J)
+            w("    if constexpr (COUNT) nv += (d%d == 0.0) ? 1 : 0;" % J)
         w("    bad |= (d%d == 0.0); lu[%d] = 1.0 / d%d;" % (J, slot[(J, J)], J))
         w("  }")
-    w("  *viol += nv;")
+    w("  if constexpr (COUNT) *viol += nv;")
     w("  return bad;")
     w("}")
     w("")

@@ -658,6 +658,33 @@ int brams_chem_rodas3_dyndt_device_noalloc(const brams_chem_dev_args* a) {
   return nfail;
 }
 
+// chem_accum on device (chemistry.f90:528-604), for callers that keep the
+// tracers in HBM: no synchronisation, stream-ordered.
+int brams_chem_accum_device(int32_t m1, int32_t m2, int32_t m3, int32_t ntask, int32_t n_dyn_chem,
+                            int32_t n_transported, const int32_t* transp_chem_index, float* sc_t_dyn,
+                            const float* sc_t, void* stream) {
+  if (m1 < 1 || m2 < 1 || m3 < 1) return fail(-2, "bad grid extents %d %d %d", m1, m2, m3);
+  if (n_dyn_chem < 1) return fail(-2, "N_DYN_CHEM must be >= 1");
+  if (n_transported < 0 || n_transported > MECH_NS) return fail(-2, "n_transported %d out of range", n_transported);
+  if (n_transported > 0 && !transp_chem_index) return fail(-2, "null transp_chem_index");
+  if (!sc_t_dyn || (ntask == 1 && !sc_t)) return fail(-2, "null device pointer");
+  brams::AccumSpecies sp{};
+  sp.n = n_transported;
+  for (int t = 0; t < n_transported; ++t) {
+    const int n = transp_chem_index[t];
+    if (n < 1 || n > MECH_NS) return fail(-2, "transp_chem_index(%d) = %d out of range", t + 1, n);
+    sp.idx[t] = n - 1;
+  }
+  if (n_transported == 0) return 0;
+  const long long n3d = (long long)m1 * m2 * m3;
+  const float ni = 1.0f / (float)n_dyn_chem;      // REAL(1./N_DYN_CHEM)
+  const unsigned gx = (unsigned)std::min<long long>((n3d + 255) / 256, 1024);
+  hipLaunchKernelGGL(brams::chem_accum, dim3(gx, (unsigned)n_transported), dim3(256), 0, (hipStream_t)stream,
+                     n3d, ni, ntask == 1 ? 1 : 0, sp, sc_t_dyn, ntask == 1 ? sc_t : sc_t_dyn);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 // Several grids in one pass (the nested grids of a model step).  Block 1 of
 // every grid runs first, in one launch (each publishes its own grid's mask),
 // then the other 65..127-cell blocks of every grid, then the 64-cell blocks

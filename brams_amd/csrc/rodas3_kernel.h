@@ -108,6 +108,12 @@ constexpr int kRodas3ManyBlocks = 4096;    // from this many blocks on
 // the surrogate step) instead of the AllocIndex partition.
 template <int WG, bool DYN, bool LIST>
 __device__ __forceinline__ void integrate_block(const StepParams& p, int b) {
+  // Sparse1.3 applies its threshold rule only where it orders, on cell 1 of
+  // block 1 (rodas3:470-506); the other blocks reuse that order unchecked.
+  // Block 1 and the 65..127-cell blocks count failing pivots over all their
+  // cells; the 64-cell and list blocks skip the count (it costs them a few
+  // percent of their time).
+  constexpr bool kCountThreshold = DYN || (WG > 64);
   int start, ncell;
   if constexpr (LIST) {                             // fallback blocks of listed cells
     start = b * WG;
@@ -201,7 +207,7 @@ __device__ __forceinline__ void integrate_block(const StepParams& p, int b) {
         for (int w = 0; w < MECH_MASK_WORDS; ++w) mw[w] = s_mask[w];
       }
       double lu[MECH_NLU];
-      const int bad = mech_factor(rk, y, igdt, mw, lu, &nviol);
+      const int bad = mech_factor<kCountThreshold>(rk, y, igdt, mw, lu, &nviol);
       const double dti = 1.0 / dt;
       // stage 1, rodas3:523-527
       double k1[MECH_NS];
@@ -312,8 +318,10 @@ __device__ __forceinline__ void integrate_block(const StepParams& p, int b) {
       }
     }
   }
-  // threshold-rule count of the block: sum over its cells
-  {
+  // threshold-rule count of the block: sum over its cells (0 where not counted)
+  if constexpr (!kCountThreshold) {
+    if (c == 0 && p.blk_thresh) p.blk_thresh[b] = 0;
+  } else {
     int v = act ? nviol : 0;
     if constexpr (DYN) {
       if (c == 0 && p.order_thresh) p.order_thresh[0] = nviol;

@@ -15,6 +15,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "generated/relacs_tuv_gen.h"
+
 namespace brams {
 
 constexpr int kSortBins = 64;
@@ -109,6 +111,23 @@ __global__ void cell_history_update(const int* __restrict__ list, int nfb, const
 __global__ void partition_starts(int nob, int base, int r, int* __restrict__ block_start) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b <= nob) block_start[b] = b * base + (b < r ? b : r);
+}
+
+// chem_accum (chemistry.f90:528-604) on the transported species of a
+// species-major [nspecies][n3d] pair: ntask 1 adds ni * sc_t to sc_t_dyn,
+// otherwise sets sc_t_dyn to zero.  REAL arithmetic with the product and the
+// sum rounded separately, as the reference's non-contracted build does.
+struct AccumSpecies {
+  int n;
+  int idx[MECH_NS];          // 0-based species of the transported list
+};
+__global__ __launch_bounds__(256) void chem_accum(long long n3d, float ni, int accumulate, AccumSpecies sp,
+                                                  float* __restrict__ sc_t_dyn, const float* __restrict__ sc_t) {
+  const long long base = (long long)sp.idx[blockIdx.y] * n3d;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n3d;
+       i += (long long)gridDim.x * blockDim.x) {
+    sc_t_dyn[base + i] = accumulate ? __fadd_rn(sc_t_dyn[base + i], __fmul_rn(ni, sc_t[base + i])) : 0.0f;
+  }
 }
 
 // out[0] = how many of flags[0..n) are nonzero; one workgroup of 256

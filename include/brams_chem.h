@@ -111,10 +111,12 @@ typedef struct brams_chem_stats {
     int64_t main_cells;          /* cells that kernel integrated */
     int64_t forced_blocks;       /* blocks that accepted a step with error > 1 at dt_min */
     float max_forced_err;        /* largest such error (rodas3:671-676 accepts it) */
-    int64_t threshold_pivots;    /* pivots, over all cells and attempts, that Sparse1.3's
-                                    threshold rule would reject: |pivot| <= RelThreshold (1e-3)
-                                    x the largest other entry of its reduced column
-                                    (spFactor.c:202-203, spAllocate.c:191) */
+    int64_t threshold_pivots;    /* pivots, over the cells and attempts of block 1 and of the
+                                    65..127-cell blocks, that Sparse1.3's threshold rule would
+                                    reject: |pivot| <= RelThreshold (1e-3) x the largest other
+                                    entry of its reduced column (spFactor.c:202-203,
+                                    spAllocate.c:191).  Sparse applies the rule only where it
+                                    orders, on block 1; the 64-cell blocks do not count */
     int64_t order_threshold_pivots; /* the same count on cell 1 of block 1, the matrix the
                                     reference orders on every block-1 attempt
                                     (chem_spack_rodas3_dyndt.f90:470-506): 0 means Sparse's
@@ -155,6 +157,20 @@ int brams_chem_rodas3_dyndt_device(const brams_chem_dev_args *args);
 int brams_chem_rodas3_dyndt_device_noalloc(const brams_chem_dev_args *args);
 
 /*
+ * chem_accum (chemistry.f90:528-604) on device arrays, for a model that keeps
+ * the tracers in HBM.  With the PARALLEL split and N_DYN_CHEM > 1 the driver
+ * accumulates the mean dynamic tendency of the transported species over a
+ * chemistry step: ntask 1 (every dynamics step) adds REAL(1./N_DYN_CHEM) *
+ * sc_t to sc_t_dyn, any other ntask (after the chemistry step) sets sc_t_dyn
+ * to zero.  sc_t, sc_t_dyn: device [nspecies][m1*m2*m3] (sc_t may be NULL
+ * when ntask != 1); transp_chem_index: host, 1-based, n_transported entries.
+ * Stream-ordered; does not synchronise.
+ */
+int brams_chem_accum_device(int32_t m1, int32_t m2, int32_t m3, int32_t ntask, int32_t n_dyn_chem,
+                            int32_t n_transported, const int32_t *transp_chem_index, float *sc_t_dyn,
+                            const float *sc_t, void *stream);
+
+/*
  * The nested grids of one model step in one pass.  chemistry_driver calls
  * chem_rodas3_dyndt once per grid (chemistry.f90:313-320); this integrates
  * grids[0..ngrids) together: block 1 of every grid first (each publishes its
@@ -189,7 +205,8 @@ int brams_chem_block_status(int32_t *attempts, int32_t *substeps, int32_t *faile
                             float *forced_err, int32_t max_nob);
 
 /* Per-block count of threshold-rule pivot failures of the last step (see
- * brams_chem_stats.threshold_pivots); no reference counterpart.  Returns nob. */
+ * brams_chem_stats.threshold_pivots; 0 for the 64-cell blocks, which do not
+ * count); no reference counterpart.  Returns nob. */
 int brams_chem_block_thresholds(int32_t *counts, int32_t max_nob);
 
 /*

@@ -97,7 +97,7 @@ int main(int argc, char** argv) {
       const double igdt = 1.0 / (0.5 * (0.5 + 119.5 * U(gen)));
       double lu0[MECH_NLU], lu1[MECH_NLU];
       int v0 = 0, v1 = 0;
-      const int b0 = mech_factor(rk0, y, igdt, mw, lu0, &v0);
+      const int b0 = mech_factor<true>(rk0, y, igdt, mw, lu0, &v0);
       double jm[MECH_NJAC];
       mech_jac_t(rk0, y, mw, jm);               // masked per call, as the slab kernel does
       const int b1 = mech_factor_rl(jm, igdt, mw, lu1, &v1);

@@ -9,7 +9,8 @@ result equals what the full reference step gives it.  Checked per sampled
 block: every cell's REAL outputs within rel 1e-6 / abs 1e-12 ppb, attempts
 and accepted sub-steps equal, last_accepted_dt within rel 1e-9 (the
 controller's err**(-1/3) carries the round-off of fused multiply-adds, about
-1e-11 relative), threshold-rule counts equal; globally the mask size.
+1e-11 relative), threshold-rule counts equal on the blocks the GPU counts
+them in (block 1 and the 65..127-cell blocks); globally the mask size.
 
 Some blocks of the plume blow up: they accept steps at dt_min whose error
 overflows (above 3.4e38), and some of them end in a non-finite state or a
@@ -115,7 +116,11 @@ def run_case(grid, sza, tol, every, head=0, seed=0):
     assert np.array_equal(gsub[ids], osub[ids])
     assert nfail == 0, worst
     np.testing.assert_allclose(gdt[ids], hs.last_dt[ids], rtol=1e-9)
-    assert np.array_equal(gth[ids], oth[ids])
+    # the GPU counts threshold-rule failures in block 1 and the 65..127-cell
+    # blocks (where Sparse1.3 orders, and the 128-lane kernel); the oracle in all
+    sizes = np.asarray(wl["block_end"])
+    counted = ids[(ids == 0) | (sizes[ids] > 64)]
+    assert np.array_equal(gth[counted], oth[counted])
     assert st.order_threshold_pivots == ost.order_threshold_pivots == 0
     return st
 
