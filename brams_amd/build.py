@@ -28,22 +28,23 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
 # in outputs and attempt counts; parity at full size is checked by the
 # 1.6M-cell tests, which report accept/reject flips separately.
 OBJ = os.path.join(PKG, "_build")
-# translation units of libbrams_chem.so.  The per-lane RODAS3 kernels take
-# about two minutes of compile each, so they are spread over three units
-# (rodas3_launch*.hip, rodas3_grids.hip) that compile in parallel; the slab
-# kernels (two minutes) live in rodas3_slab.hip.
-HIP_UNITS = ["rodas3_launch.hip", "rodas3_launch_b.hip", "rodas3_grids.hip", "rodas3_slab.hip",
-             "brams_chem.hip", "surrogate.hip"]
+# translation units of libbrams_chem.so.  The per-lane RODAS3 and ROS2 kernels
+# take about two minutes of compile each, so they are spread over four units
+# (rodas3_launch*.hip, rodas3_grids.hip, ros2_launch.hip) that compile in
+# parallel; the slab kernels (two minutes) live in rodas3_slab.hip.
+HIP_UNITS = ["rodas3_launch.hip", "rodas3_launch_b.hip", "rodas3_grids.hip", "ros2_launch.hip",
+             "rodas3_slab.hip", "brams_chem.hip", "surrogate.hip"]
 _RODAS3_DEPS = ["rodas3_kernel.h", "rodas3_launch.h", "mech_families.h", "generated/relacs_tuv_gen.h"]
 UNIT_DEPS = {
     "rodas3_launch.hip": _RODAS3_DEPS,
     "rodas3_launch_b.hip": _RODAS3_DEPS,
     "rodas3_grids.hip": _RODAS3_DEPS,
+    "ros2_launch.hip": _RODAS3_DEPS,
     "rodas3_slab.hip": ["rodas3_slab.h", "rodas3_slab_launch.h", "rodas3_kernel.h", "mech_families.h",
                         "generated/relacs_tuv_gen.h", "generated/relacs_tuv_slab.h"],
 }
 # parallel compiles (each needs a few GB of host memory)
-HIP_JOBS = int(os.environ.get("BRAMS_HIP_JOBS", "4"))
+HIP_JOBS = int(os.environ.get("BRAMS_HIP_JOBS", "6"))
 REF_ROOT = os.environ.get("BRAMS_REFERENCE", "/root/reference")
 
 

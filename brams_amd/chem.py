@@ -4,8 +4,10 @@ chem_rodas3_dyndt() keeps the argument names, meaning and units of
 mod_chem_spack_rodas3_dyndt::chem_rodas3_dyndt
 (src/brams/ccatt/chem_spack_rodas3_dyndt.f90:41-131) for the arguments the
 RELACS_TUV path reads; it updates chem1_sc_p (or sc_t) and last_accepted_dt in
-place like the Fortran routine.  DeviceStep runs the same step on arrays that
-already live in HBM (torch tensors are used only as device memory).
+place like the Fortran routine.  chem_ros_dyndt() is the same for the ROS2
+solver (chem_spack_ros_dyndt.f90:39-45, CHEMISTRY=3).  DeviceStep runs either
+step on arrays that already live in HBM (torch tensors are used only as
+device memory).
 """
 import ctypes as C
 
@@ -34,14 +36,19 @@ def alloc_index(m1, ia, iz, ja, jz, block_size=64):
     return out
 
 
+SOLVERS = {"rodas3": "brams_chem_rodas3_dyndt", "ros2": "brams_chem_ros_dyndt"}
+
+
 def chem_rodas3_dyndt(m1, m2, m3, ia, iz, ja, jz, dtlt, pp, pi0, theta, rv, nspecies, nr, nr_photo,
                       weight, PhotojMethod, transp_chem_index, no_transp_chem_index, chem1_sc_p,
                       nob, maxblock_size, block_end, last_accepted_dt, atol, rtol, N_DYN_CHEM,
-                      split_method, cp, cpor, p00, jphoto, sc_t=None, sc_t_dyn=None, device=0):
+                      split_method, cp, cpor, p00, jphoto, sc_t=None, sc_t_dyn=None, device=0,
+                      solver="rodas3"):
     """One chemistry step on one grid; arrays are updated in place.
 
     Returns the number of blocks with a numerical failure (the reference would
-    abort there); raises abi.ChemError on invalid arguments or a HIP error."""
+    abort there); raises abi.ChemError on invalid arguments or a HIP error.
+    solver "ros2" runs chem_ros_dyndt's method instead (chem_ros_dyndt())."""
     L = abi.lib()
     abi.check(L.brams_chem_init(device), "brams_chem_init")
     keep = []
@@ -83,12 +90,20 @@ def chem_rodas3_dyndt(m1, m2, m3, ia, iz, ja, jz, dtlt, pp, pi0, theta, rv, nspe
     rt = np.ascontiguousarray(rtol, np.float64); keep.append(rt)
     a.atol = at.ctypes.data_as(C.POINTER(C.c_double))
     a.rtol = rt.ctypes.data_as(C.POINTER(C.c_double))
-    return abi.check(L.brams_chem_rodas3_dyndt(C.byref(a)), "brams_chem_rodas3_dyndt")
+    name = SOLVERS[solver]
+    return abi.check(getattr(L, name)(C.byref(a)), name)
+
+
+def chem_ros_dyndt(*args, **kw):
+    """chem_ros_dyndt (chem_spack_ros_dyndt.f90:39-45): the arguments of
+    chem_rodas3_dyndt(), integrated with the ROS2 method."""
+    return chem_rodas3_dyndt(*args, solver="ros2", **kw)
 
 
 def run_workload_host(wl, atol, rtol, split_method="SYMMETRIC", n_dyn_chem=1, sc_t=None,
-                      sc_t_dyn=None):
-    """Convenience: run chem_rodas3_dyndt on a brams_amd.workload dict (copies)."""
+                      sc_t_dyn=None, solver="rodas3"):
+    """Convenience: run chem_rodas3_dyndt (or chem_ros_dyndt) on a
+    brams_amd.workload dict (copies)."""
     mech = load_mechanism()
     g = wl["grid"]
     sc_p = [np.array(x, copy=True, order="F") for x in wl["sc_p"]]
@@ -101,7 +116,7 @@ def run_workload_host(wl, atol, rtol, split_method="SYMMETRIC", n_dyn_chem=1, sc
                            [s.weight for s in mech.species], "FAST-TUV", tidx, nidx, sc_p,
                            len(wl["block_end"]), int(max(wl["block_end"])), wl["block_end"],
                            last_dt, atol, rtol, n_dyn_chem, split_method, 1004.0, 1004.0 / 287.0,
-                           1.0e5, wl["jphoto"], sc_t=sct, sc_t_dyn=sc_t_dyn)
+                           1.0e5, wl["jphoto"], sc_t=sct, sc_t_dyn=sc_t_dyn, solver=solver)
     return sc_p, last_dt, rc, sct
 
 
@@ -112,7 +127,7 @@ class DeviceStep:
     resident state (sc_p and last_accepted_dt are updated in place on the
     device, as a time loop would)."""
 
-    def __init__(self, wl, atol, rtol, device=0, split_method="SYMMETRIC", n_dyn_chem=1):
+    def __init__(self, wl, atol, rtol, device=0, split_method="SYMMETRIC", n_dyn_chem=1, solver="rodas3"):
         import torch
         self.torch = torch
         L = abi.lib()
@@ -159,6 +174,7 @@ class DeviceStep:
         a.stream = torch.cuda.current_stream(dev).cuda_stream
         self.args = a
         self.ncell = g.ncell
+        self.entry = SOLVERS[solver] + "_device"
 
     def reset(self):
         """Restore the initial state (for repeated timing of the same step)."""
@@ -166,9 +182,7 @@ class DeviceStep:
         self.last_dt.copy_(self.last_dt0)
 
     def step(self):
-        rc = abi.check(abi.lib().brams_chem_rodas3_dyndt_device(C.byref(self.args)),
-                       "brams_chem_rodas3_dyndt_device")
-        return rc
+        return abi.check(getattr(abi.lib(), self.entry)(C.byref(self.args)), self.entry)
 
     def species_host(self):
         """sc_p back on the host as a list of Fortran-ordered (m1,m2,m3) arrays."""

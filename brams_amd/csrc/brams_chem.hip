@@ -433,6 +433,63 @@ int launch_step(brams::StepParams p, hipStream_t st) {
   return 0;
 }
 
+// The mask a ROS2 step reports: every structural entry, bits 0..MECH_NSTRUCT-1
+// (ROS2 factors the full pattern, chem_spack_ros_dyndt.f90 has no value mask).
+struct StructMask {
+  unsigned w[MECH_MASK_WORDS];
+};
+constexpr StructMask struct_mask() {
+  StructMask m{};
+  for (int w = 0; w < MECH_MASK_WORDS; ++w) {
+    const int bits = MECH_NSTRUCT - 32 * w;
+    m.w[w] = bits >= 32 ? 0xffffffffu : bits > 0 ? (1u << bits) - 1u : 0u;
+  }
+  return m;
+}
+constexpr StructMask kStructMask = struct_mask();
+
+// The ROS2 solver (CHEMISTRY=3, chem_spack_ros_dyndt.f90) on the current
+// grid: every block alike on the full structural Jacobian, so no block-1 form
+// and no mask; the 65..127-cell blocks on 128 lanes, the 64-cell blocks
+// longest first from the second call on the grid.
+int launch_ros2_step(brams::StepParams p, hipStream_t st) {
+  GridState& gs = *g.cur;
+  const int nob = (int)gs.sizes.size();
+  p.block_start = gs.block_start.as<int>();
+  p.mask = gs.mask.as<unsigned>();
+  p.blk_attempts = gs.attempts.as<int>();
+  p.blk_substeps = gs.substeps.as<int>();
+  p.blk_failed = gs.failed.as<int>();
+  p.blk_forced = gs.forced.as<float>();
+  p.blk_thresh = gs.thresh.as<int>();
+  p.order_thresh = nullptr;
+  HIP_TRY(hipMemcpyAsync(gs.mask.p, kStructMask.w, sizeof kStructMask.w, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemsetAsync(gs.order_thresh.p, 0, sizeof(int), st));
+  int nbig = 0;
+  while (nbig < nob && gs.sizes[nbig] > 64) ++nbig;
+  const int* order = nullptr;
+  if (nob > nbig && gs.blk_hist) {
+    int rc = order_blocks(nbig, nob - nbig, st);
+    if (rc) return rc;
+    order = g.order.as<int>();
+  }
+  HIP_TRY(hipEventRecord(g.ev0, st));
+  HIP_TRY(brams::launch_ros2(128, (unsigned)nbig, p, 0, st));
+  HIP_TRY(hipEventRecord(g.evb, st));
+  HIP_TRY(hipEventRecord(g.evm, st));
+  g.main_cells = 0;
+  if (nob > nbig) {
+    brams::StepParams pm = p;
+    pm.block_order = order;
+    HIP_TRY(brams::launch_ros2(64, (unsigned)(nob - nbig), pm, nbig, st));
+    for (int b = nbig; b < nob; ++b) g.main_cells += gs.sizes[b];
+  }
+  HIP_TRY(hipEventRecord(g.ev1, st));
+  gs.blk_hist = true;
+  g.last_form = -1;
+  return 0;
+}
+
 int collect_stats(hipStream_t st) {
   GridState& gs = *g.cur;
   const int nob = (int)gs.sizes.size();
@@ -1013,7 +1070,34 @@ void brams_sur_reset(void) {
   for (GridState* gs : g.grids) gs->hist_cells = -1;
 }
 
-int brams_chem_rodas3_dyndt(const brams_chem_args* a) {
+}  // extern "C"
+
+namespace {
+int host_step(const brams_chem_args* a, int solver);
+}  // namespace
+
+extern "C" {
+
+int brams_chem_rodas3_dyndt(const brams_chem_args* a) { return host_step(a, brams::kRodas3); }
+
+int brams_chem_ros_dyndt(const brams_chem_args* a) { return host_step(a, brams::kRos2); }
+
+int brams_chem_ros_dyndt_device(const brams_chem_dev_args* a) {
+  brams::StepParams p;
+  long long ncell = 0;
+  int rc = prepare_device(a, p, ncell);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)a->stream;
+  if ((rc = launch_ros2_step(p, st))) return rc;
+  return collect_stats(st);
+}
+
+}  // extern "C"
+
+namespace {
+
+// one chemistry step with host arrays (chem_rodas3_dyndt / chem_ros_dyndt)
+int host_step(const brams_chem_args* a, int solver) {
   if (!a) return fail(-2, "null args");
   if (!g.init) {
     int rc = brams_chem_init(0);
@@ -1110,7 +1194,7 @@ int brams_chem_rodas3_dyndt(const brams_chem_args* a) {
   p.sc_t_dyn = need_td ? g.sc_t_dyn.as<float>() : nullptr;
   p.n3d = n3d;
   p.last_dt = g.last_dt.as<double>();
-  if ((rc = launch_step(p, st))) return rc;
+  if ((rc = solver == brams::kRos2 ? launch_ros2_step(p, st) : launch_step(p, st))) return rc;
   // results back into the caller's arrays
   if (need_t) {
     for (int s = 0; s < MECH_NS; ++s)
@@ -1123,6 +1207,10 @@ int brams_chem_rodas3_dyndt(const brams_chem_args* a) {
                          hipMemcpyDeviceToHost, st));
   return collect_stats(st);
 }
+
+}  // namespace
+
+extern "C" {
 
 int brams_chem_block_status(int32_t* attempts, int32_t* substeps, int32_t* failed,
                             float* forced_err, int32_t max_nob) {

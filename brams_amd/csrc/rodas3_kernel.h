@@ -104,10 +104,22 @@ __device__ __forceinline__ void block_reduce(double& errmax, int& rej, int& fail
 constexpr int kRodas3ManyWaves = 4;        // WPS of large launches
 constexpr int kRodas3ManyBlocks = 4096;    // from this many blocks on
 
+// The two Rosenbrock solvers of the reference on the same mechanism code:
+// RODAS3 (chem_spack_rodas3_dyndt.f90, CHEMISTRY=4) and the two-stage ROS2
+// (chem_spack_ros_dyndt.f90, CHEMISTRY=3).
+enum { kRodas3 = 0, kRos2 = 1 };
+// ROS2's gamma = 1 + 1/sqrt(2), with sqrt(2) written as the double value of
+// its single-precision rounding, 1.4142135381698608 (ros_dyndt:130)
+constexpr double kRos2Gamma = 1.0 + 1.0 / 1.4142135381698608;
+constexpr double kRos2GammaInv = 1.0 / kRos2Gamma;
+
 // LIST: the blocks are consecutive WG-sized runs of p.cell_list (fallback of
 // the surrogate step) instead of the AllocIndex partition.
-template <int WG, bool DYN, bool LIST>
+// SOLVER kRos2: chem_ros_dyndt's integration, which uses the full structural
+// Jacobian (no value mask, so no block-1 form) and no clipping.
+template <int WG, bool DYN, bool LIST, int SOLVER = kRodas3>
 __device__ __forceinline__ void integrate_block(const StepParams& p, int b) {
+  static_assert(SOLVER == kRodas3 || (!DYN && !LIST), "ROS2 has no block-1 or list form");
   // Sparse1.3 applies its threshold rule only where it orders, on cell 1 of
   // block 1 (rodas3:470-506); the other blocks reuse that order unchecked.
   // Block 1 and the 65..127-cell blocks count failing pivots over all their
@@ -182,7 +194,10 @@ __device__ __forceinline__ void integrate_block(const StepParams& p, int b) {
   __shared__ double s_err[WG / 64 + 1];
   __shared__ int s_rej[WG / 64 + 1], s_fail[WG / 64 + 1];
   unsigned mw[MECH_MASK_WORDS];
-  if constexpr (!DYN) {
+  if constexpr (SOLVER == kRos2) {
+#pragma unroll
+    for (int w = 0; w < MECH_MASK_WORDS; ++w) mw[w] = 0xffffffffu;   // every structural entry
+  } else if constexpr (!DYN) {
 #pragma unroll
     for (int w = 0; w < MECH_MASK_WORDS; ++w) mw[w] = p.mask[w];
   }
@@ -192,7 +207,8 @@ __device__ __forceinline__ void integrate_block(const StepParams& p, int b) {
     mech_fun(rk, y, f0);
     for (;;) {
       ++natt;
-      const double igdt = 1.0 / (0.5 * dt);
+      // 1/(gamma h): RODAS3 gamma = 1/2 (rodas3:462), ROS2 ros_dyndt:387
+      const double igdt = SOLVER == kRos2 ? 1.0 / (kRos2Gamma * dt) : 1.0 / (0.5 * dt);
       if constexpr (DYN) {
         // every lane evaluates the mask of its own cell, so the evaluation
         // is no divergent region; cell 1's is the one published
@@ -209,6 +225,31 @@ __device__ __forceinline__ void integrate_block(const StepParams& p, int b) {
       double lu[MECH_NLU];
       const int bad = mech_factor<kCountThreshold>(rk, y, igdt, mw, lu, &nviol);
       const double dti = 1.0 / dt;
+      double yn[MECH_NS];
+      double e = 0.0;
+      if constexpr (SOLVER == kRos2) {
+        // stage 1, ros_dyndt:404-424: k1 = (1/(gamma h) - J)^-1 f(y0)
+        double k1[MECH_NS], k2[MECH_NS];
+#pragma unroll
+        for (int s = 0; s < MECH_NS; ++s) k1[s] = f0[s];
+        mech_solve(lu, k1);
+#pragma unroll
+        for (int s = 0; s < MECH_NS; ++s) yn[s] = y[s] + kRos2GammaInv * k1[s];
+        // stage 2, ros_dyndt:433-460: f(y0 + k1/gamma) - 2/(gamma h) k1
+        mech_fun(rk, yn, k2);
+#pragma unroll
+        for (int s = 0; s < MECH_NS; ++s) k2[s] = k2[s] - 2.0 * kRos2GammaInv * dti * k1[s];
+        mech_solve(lu, k2);
+        // solution and error, ros_dyndt:463-498 (no clipping)
+#pragma unroll
+        for (int s = 0; s < MECH_NS; ++s) {
+          const double v = y[s] + kRos2GammaInv * (1.5 * k1[s] + 0.5 * k2[s]);
+          yn[s] = v;
+          const double tol = p.atol[s] + p.rtol[s] * fmax(fabs(y[s]), fabs(v));
+          const double q = (0.5 * kRos2GammaInv * (k1[s] + k2[s])) / tol;
+          e = e + q * q;
+        }
+      } else {
       // stage 1, rodas3:523-527
       double k1[MECH_NS];
 #pragma unroll
@@ -220,7 +261,7 @@ __device__ __forceinline__ void integrate_block(const StepParams& p, int b) {
       for (int s = 0; s < MECH_NS; ++s) k2[s] = (4.0 * dti) * k1[s] + f0[s];
       mech_solve(lu, k2);
       // stage 3 with clipping, rodas3:542-566
-      double yn[MECH_NS], ft[MECH_NS];
+      double ft[MECH_NS];
 #pragma unroll
       for (int s = 0; s < MECH_NS; ++s) {
         yn[s] = y[s] + 2.0 * k1[s];
@@ -251,7 +292,6 @@ __device__ __forceinline__ void integrate_block(const StepParams& p, int b) {
       for (int s = 0; s < MECH_NS; ++s) ft[s] = ft[s] + dti * (k2[s] - c83 * k3[s]);
       mech_solve(lu, ft);                          // ft now holds k4
       // solution and error, rodas3:609-648
-      double e = 0.0;
 #pragma unroll
       for (int s = 0; s < MECH_NS; ++s) {
         double v = y[s] + 2.0 * k1[s] + k3[s] + ft[s];
@@ -261,6 +301,7 @@ __device__ __forceinline__ void integrate_block(const StepParams& p, int b) {
         const double q = ft[s] / tol;
         e = e + q * q;
       }
+      }
       const double err = fmax(1.0e-15, sqrt(e / (double)MECH_NS));
       double errmax = act ? err : 0.0;
       int rej = (act && (err - 1.0e-8 > 1.0)) ? 1 : 0;
@@ -269,8 +310,8 @@ __device__ __forceinline__ void integrate_block(const StepParams& p, int b) {
       int fail = (act && (bad || !isfinite(e))) ? 1 : 0;
       if (natt >= kMaxAttempts) fail = 1;
       block_reduce<WG>(errmax, rej, fail, s_err, s_rej, s_fail);
-      // controller, rodas3:650-700
-      const double fac = fmin(6.0, fmax(0.2, 0.9 / pow(errmax, 1.0 / 3.0)));
+      // controller, rodas3:650-700 (error order 3) and ros_dyndt:501-551 (order 2)
+      const double fac = fmin(6.0, fmax(0.2, 0.9 / pow(errmax, SOLVER == kRos2 ? 1.0 / 2.0 : 1.0 / 3.0)));
       double dt_new = dt * fac;
       dt_new = fmax(dt_min, fmin(dt_max, dt_new));
       if (fail) {
@@ -355,6 +396,13 @@ template <int WG, bool DYN, bool LIST, int WPS>
 __global__ __launch_bounds__(WG, WPS) void rodas3_blocks(StepParams p, int block0) {
   const int b = (!LIST && p.block_order) ? p.block_order[blockIdx.x] : block0 + blockIdx.x;
   integrate_block<WG, DYN, LIST>(p, b);
+}
+
+// CHEMISTRY=3: chem_ros_dyndt's blocks, all of one launch alike (no block-1 form)
+template <int WG, int WPS>
+__global__ __launch_bounds__(WG, WPS) void ros2_blocks(StepParams p, int block0) {
+  const int b = p.block_order ? p.block_order[blockIdx.x] : block0 + blockIdx.x;
+  integrate_block<WG, false, false, kRos2>(p, b);
 }
 
 // Several grids in one launch (the nested grids of one model step, which the

@@ -1,6 +1,6 @@
-// Host entry of the RODAS3 block kernels (defined in rodas3_launch.hip,
-// rodas3_launch_b.hip and rodas3_grids.hip, split so the units compile in
-// parallel).
+// Host entry of the RODAS3 and ROS2 block kernels (defined in
+// rodas3_launch.hip, rodas3_launch_b.hip, rodas3_grids.hip and
+// ros2_launch.hip, split so the units compile in parallel).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -20,6 +20,11 @@ hipError_t launch_rodas3_list(unsigned nblocks, const StepParams& p, hipStream_t
 // the parts of the above that live in rodas3_launch_b.hip
 hipError_t launch_rodas3_128(bool dyn, unsigned nblocks, const StepParams& p, int block0, hipStream_t st);
 hipError_t launch_rodas3_64w2(unsigned nblocks, const StepParams& p, int block0, hipStream_t st);
+
+// The ROS2 solver (ros2_launch.hip): nblocks workgroups of wg (64 or 128)
+// lanes over blocks block0.. of p; the 64-lane form takes four waves per SIMD
+// from kRodas3ManyBlocks blocks on, else one.
+hipError_t launch_ros2(int wg, unsigned nblocks, const StepParams& p, int block0, hipStream_t st);
 
 // Several grids in one launch (rodas3_grids): nwork workgroups of wg lanes,
 // workgroup w on block work[w] & kGridBlockMask of grid work[w] >> kGridShift,

@@ -140,6 +140,19 @@ int brams_chem_rodas3_dyndt(const brams_chem_args *args);
 int brams_chem_rodas3_dyndt_device(const brams_chem_dev_args *args);
 
 /*
+ * Drop-in for chem_ros_dyndt (chem_spack_ros_dyndt.f90:39-45, CHEMISTRY=3):
+ * the same step with the two-stage Rosenbrock method of that file (gamma =
+ * 1 + 1/sqrt(2), error order 2), on the full structural Jacobian (no value
+ * mask) and without clipping of the stages.  Arguments, units, status and
+ * write-back as brams_chem_rodas3_dyndt; threshold counts are kept for the
+ * 65..127-cell blocks only.  Host arrays.
+ */
+int brams_chem_ros_dyndt(const brams_chem_args *args);
+
+/* Device-resident variant of brams_chem_ros_dyndt (same semantics). */
+int brams_chem_ros_dyndt_device(const brams_chem_dev_args *args);
+
+/*
  * No counterpart in the reference.  brams_chem_rodas3_dyndt_device for
  * callers that cannot allocate device memory (code under stream capture,
  * tight serving loops, harnesses that forbid hipMalloc): the same step

@@ -434,7 +434,10 @@ typedef struct {
     long *sp_elem;
     int *sp_ipos, *sp_jpos;
     int sp_nnz;
+    int solver;                  /* ORA_RODAS3 or ORA_ROS2 */
 } drv_t;
+
+enum { ORA_RODAS3 = 0, ORA_ROS2 = 1 };
 
 typedef struct {
     int n;
@@ -604,6 +607,52 @@ double ora_dt_new(double dt, double max_err, double dt_min, double dt_max)
     return rodas3_dt_new(dt, max_err, dt_min, dt_max);
 }
 
+/* ------------------------------------------------------------------ ROS2
+ * chem_spack_ros_dyndt.f90 (CHEMISTRY=3): the two-stage Rosenbrock method
+ * with gamma = 1 + 1/sqrt(2); the reference writes sqrt(2) as the double
+ * value of its single-precision rounding (ros_dyndt:130). */
+static const double ros2_gamma = 1.0 + 1.0 / 1.4142135381698608;
+
+/* The two stages and the solution of one attempt for one cell,
+ * ros_dyndt:404-477 (no clipping); returns the error sum of squares with
+ * err_k = 0.5/gamma (k1 + k2), ros_dyndt:486-498. */
+static double ros2_attempt(int n, void *ctx, stage_fun_fn fun, stage_solve_fn solve, const double *y0,
+                           const double *f0, double dt, const double *atol, const double *rtol,
+                           double *k1, double *k2, double *yn, double *err)
+{
+    const double gi = 1.0 / ros2_gamma;
+    const double dti = 1.0 / dt;
+    double ft[NS], bb[NS];
+    solve(ctx, f0, k1);                                         /* ros_dyndt:404-410 */
+    for (int s = 0; s < n; s++) yn[s] = y0[s] + gi * k1[s];     /* ros_dyndt:418-424 */
+    fun(ctx, yn, ft);                                           /* ros_dyndt:433-437 */
+    for (int s = 0; s < n; s++) bb[s] = ft[s] - 2.0 * gi * dti * k1[s];   /* ros_dyndt:440-445 */
+    solve(ctx, bb, k2);                                         /* ros_dyndt:454-460 */
+    double e = 0.0;
+    for (int s = 0; s < n; s++) {
+        double v = y0[s] + gi * (1.5 * k1[s] + 0.5 * k2[s]);    /* ros_dyndt:463-477 */
+        yn[s] = v;
+        double tol = atol[s] + rtol[s] * fmax(fabs(y0[s]), fabs(v));
+        double q = (0.5 * gi * (k1[s] + k2[s])) / tol;
+        e = e + q * q;
+    }
+    *err = fmax(1.0e-15, sqrt(e / n));
+    return e;
+}
+
+/* New step, ros_dyndt:513-517: dt * min(6, max(0.2, 0.9 err^-1/2)) clamped */
+static double ros2_dt_new(double dt, double max_err, double dt_min, double dt_max)
+{
+    double fac = fmin(6.0, fmax(0.2, 0.9 / pow(max_err, 1.0 / 2.0)));
+    double dt_new = dt * fac;
+    return fmax(dt_min, fmin(dt_max, dt_new));
+}
+
+double ora_ros2_dt_new(double dt, double max_err, double dt_min, double dt_max)
+{
+    return ros2_dt_new(dt, max_err, dt_min, dt_max);
+}
+
 typedef struct { drv_t *D; lumat *lu; const double *rk; } stage_ctx_t;
 
 static void stage_fun(void *vctx, const double *y, double *f)
@@ -689,6 +738,46 @@ int ora_rodas3_linear(int n, const double *A, const double *y0, double h, int ns
         lin_fun(&c, y, f0);
         rodas3_stages(n, &c, lin_fun, lin_solve, y, f0, h, k1, k2, k3, k4, yn);
         rodas3_solution(n, y, k1, k3, k4, ones, ones, yn, &err);
+        for (int i = 0; i < n; i++) y[i] = yn[i];
+    }
+    for (int i = 0; i < n; i++) y_out[i] = y[i];
+    if (err_out) *err_out = err;
+    return 0;
+}
+
+/* ROS2 on the linear test problem y' = A y, as ora_rodas3_linear */
+int ora_ros2_linear(int n, const double *A, const double *y0, double h, int nsteps,
+                    double *y_out, double *err_out)
+{
+    if (n < 1 || n > NS) return -1;
+    lin_ctx_t c;
+    c.n = n;
+    c.A = A;
+    for (int i = 0; i < n; i++) {
+        c.piv[i] = i;
+        for (int j = 0; j < n; j++) c.lu[i * n + j] = (i == j ? 1.0 / (ros2_gamma * h) : 0.0) - A[i * n + j];
+    }
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        for (int i = k + 1; i < n; i++)
+            if (fabs(c.lu[i * n + k]) > fabs(c.lu[p * n + k])) p = i;
+        if (c.lu[p * n + k] == 0.0) return 1;
+        if (p != k) {
+            for (int j = 0; j < n; j++) {
+                double t = c.lu[k * n + j]; c.lu[k * n + j] = c.lu[p * n + j]; c.lu[p * n + j] = t;
+            }
+            int t = c.piv[k]; c.piv[k] = c.piv[p]; c.piv[p] = t;
+        }
+        for (int i = k + 1; i < n; i++) {
+            c.lu[i * n + k] /= c.lu[k * n + k];
+            for (int j = k + 1; j < n; j++) c.lu[i * n + j] -= c.lu[i * n + k] * c.lu[k * n + j];
+        }
+    }
+    double y[NS], f0[NS], k1[NS], k2[NS], yn[NS], ones[NS], err = 0.0;
+    for (int i = 0; i < n; i++) { y[i] = y0[i]; ones[i] = 1.0; }
+    for (int s = 0; s < nsteps; s++) {
+        lin_fun(&c, y, f0);
+        ros2_attempt(n, &c, lin_fun, lin_solve, y, f0, h, ones, ones, k1, k2, yn, &err);
         for (int i = 0; i < n; i++) y[i] = yn[i];
     }
     for (int i = 0; i < n; i++) y_out[i] = y[i];
@@ -842,12 +931,99 @@ static int integrate_block(drv_t *D, blk_t *B, int b)
     return failed;
 }
 
+/* ROS2 on one block, chem_spack_ros_dyndt.f90:340-555.  The matrix is
+ * 1/(gamma h) I - J on every structural entry (get_always_zeros,
+ * chem_spack_ros.f90:578-627, keeps the nonzeros of J + I at a dummy state,
+ * which are the structural ones); with the reference's Sparse1.3 the matrix
+ * is created once and its first factorisation orders (chem_spack_solve_sparse.f90). */
+static int integrate_block_ros2(drv_t *D, blk_t *B, int b)
+{
+    const brams_chem_args *a = D->a;
+    const float tn = a->dtlt * (float)a->n_dyn_chem;     /* REAL product */
+    double dt = a->last_accepted_dt[b];
+    const double dt_min = fmax(1.0, 1.0e-2 * (double)tn);
+    const double dt_max = (double)tn;
+    double time_c = 0.0;
+    const double time_f = (double)tn;
+    int failed = 0, viol = 0;
+    int64_t natt = 0, nsub = 0;
+    float forced = 0.f;
+
+    for (int c = 0; c < B->n; c++)
+        ora_kinetic(B->temp[c], B->press[c], B->vapp[c], B->jph[c], B->rk[c]);
+
+    while (time_c + 1.0e-8 < time_f) {                   /* ros_dyndt:348 */
+        for (int c = 0; c < B->n; c++) {                 /* ros_dyndt:351-362, once per sub-step */
+            ora_jacobian(B->rk[c], B->y[c], B->jac[c]);
+            ora_fexchem(B->rk[c], B->y[c], B->f0[c]);
+        }
+        atomic_fetch_add(&g_jac_calls, B->n);
+        atomic_fetch_add(&g_f0_calls, B->n);
+        for (;;) {
+            natt++;
+            const double igdt = 1.0 / (ros2_gamma * dt);   /* ros_dyndt:387 */
+            for (int c = 0; c < B->n; c++) {
+                if (D->has_sparse) {
+                    if (!D->sp_handle) {
+                        int nz = 0;
+                        for (int i = 0; i < NS; i++)
+                            for (int j = 0; j < NS; j++)
+                                if (D->mask[i * NS + j]) { D->sp_ipos[nz] = i + 1; D->sp_jpos[nz] = j + 1; nz++; }
+                        D->sp_nnz = nz;
+                        D->sp_handle = g_sp_create(NS, nz, D->sp_ipos, D->sp_jpos, D->sp_elem);
+                    }
+                    double vals[NS * NS];
+                    for (int z = 0; z < D->sp_nnz; z++) {
+                        int i = D->sp_ipos[z] - 1, j = D->sp_jpos[z] - 1;
+                        vals[z] = (i == j) ? igdt - B->jac[c][i * NS + j] : -B->jac[c][i * NS + j];
+                    }
+                    if (g_sp_load_factor(D->sp_handle, D->sp_nnz, D->sp_elem, vals) != 0) failed = 1;
+                } else {
+                    int v = 0;
+                    ora_lu_load_jac(&B->lu, B->jac[c], igdt, D->mask);
+                    if (ora_lu_factor(&B->lu, &v) != 0) failed = 1;
+                    viol += v;
+                }
+                stage_ctx_t ctx = {D, &B->lu, B->rk[c]};
+                double e = ros2_attempt(NS, &ctx, stage_fun, stage_solve, B->y[c], B->f0[c], dt, a->atol,
+                                        a->rtol, B->k1[c], B->k2[c], B->ynew[c], &B->err[c]);
+                if (!isfinite(e)) failed = 1;
+            }
+            double max_err = 0.0;
+            int all_accepted = 1;
+            for (int c = 0; c < B->n; c++) {             /* ros_dyndt:501-510 */
+                if (B->err[c] - 1.0e-8 > 1.0) all_accepted = 0;
+                if (c == 0 || B->err[c] > max_err) max_err = B->err[c];
+            }
+            double dt_new = ros2_dt_new(dt, max_err, dt_min, dt_max);
+            if (failed) { time_c = time_f; break; }
+            if (!all_accepted && dt_new > dt_min) { dt = dt_new; continue; }   /* ros_dyndt:523-526 */
+            if (!all_accepted && max_err > forced) forced = (float)max_err;
+            time_c = time_c + dt;                        /* ros_dyndt:528-548 */
+            dt = fmin(dt_new, time_f - time_c);
+            if (time_c < time_f) a->last_accepted_dt[b] = dt_new;
+            for (int c = 0; c < B->n; c++) memcpy(B->y[c], B->ynew[c], sizeof(double) * NS);
+            nsub++;
+            break;
+        }
+    }
+    D->attempts[b] = natt;
+    D->substeps[b] = nsub;
+    D->failed[b] = failed;
+    D->forced[b] = forced;
+    D->thresh[b] = viol;
+    return failed;
+}
+
 static void run_block(drv_t *D, blk_t *B, int b)
 {
     B->n = D->a->block_end[b];
     for (int c = 0; c < B->n; c++) cell_offset(D->a, D->bstart[b] + c, &B->off[c]);
     gather(D->a, B);
-    integrate_block(D, B, b);
+    if (D->solver == ORA_ROS2)
+        integrate_block_ros2(D, B, b);
+    else
+        integrate_block(D, B, b);
     scatter(D->a, B);
 }
 
@@ -873,7 +1049,7 @@ static void *worker(void *arg)
 }
 
 static int run_step(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
-                    unsigned char *mask_out, const int32_t *list, int nlist)
+                    unsigned char *mask_out, const int32_t *list, int nlist, int solver)
 {
     ora_init();
     if (a->nspecies != NS || a->nr != NR || a->nr_photo != NP) return -1;
@@ -896,6 +1072,8 @@ static int run_step(const brams_chem_args *a, int nthreads, brams_chem_stats *st
     D->sp_elem = (long *)malloc(sizeof(long) * NS * NS);
     D->sp_ipos = (int *)malloc(sizeof(int) * NS * NS);
     D->sp_jpos = (int *)malloc(sizeof(int) * NS * NS);
+    D->solver = solver;
+    if (solver == ORA_ROS2) memcpy(D->mask, g_struct, sizeof D->mask);   /* every structural entry */
     atomic_store(&g_jac_calls, 0);
     atomic_store(&g_f0_calls, 0);
 
@@ -974,14 +1152,26 @@ static int run_step(const brams_chem_args *a, int nthreads, brams_chem_stats *st
 int ora_rodas3_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
                      unsigned char *mask_out)
 {
-    return run_step(a, nthreads, st, mask_out, NULL, 0);
+    return run_step(a, nthreads, st, mask_out, NULL, 0, ORA_RODAS3);
 }
 
 int ora_rodas3_dyndt_blocks(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
                             unsigned char *mask_out, const int32_t *blocks, int nblocks)
 {
     static const int32_t none[1] = {0};
-    return run_step(a, nthreads, st, mask_out, blocks ? blocks : none, blocks ? nblocks : 0);
+    return run_step(a, nthreads, st, mask_out, blocks ? blocks : none, blocks ? nblocks : 0, ORA_RODAS3);
+}
+
+int ora_ros2_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *st, unsigned char *mask_out)
+{
+    return run_step(a, nthreads, st, mask_out, NULL, 0, ORA_ROS2);
+}
+
+int ora_ros2_dyndt_blocks(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
+                          unsigned char *mask_out, const int32_t *blocks, int nblocks)
+{
+    static const int32_t none[1] = {0};
+    return run_step(a, nthreads, st, mask_out, blocks ? blocks : none, blocks ? nblocks : 0, ORA_ROS2);
 }
 
 int ora_block_counts(int64_t *attempts, int64_t *substeps, int max_nob)

@@ -65,6 +65,18 @@ int ora_rodas3_linear(int n, const double *A, const double *y0, double h, int ns
 /* The step-size controller: dt * min(6, max(0.2, 0.9 err^-1/3)) in [dt_min, dt_max]. */
 double ora_dt_new(double dt, double max_err, double dt_min, double dt_max);
 
+/* The ROS2 solver of chem_spack_ros_dyndt.f90 (CHEMISTRY=3): the whole step,
+ * and block 0 plus the listed blocks (blocks are independent: ROS2 has no
+ * value mask).  mask_out receives the structural pattern it factors on. */
+int ora_ros2_dyndt(const brams_chem_args *a, int nthreads, brams_chem_stats *st, unsigned char *mask_out);
+int ora_ros2_dyndt_blocks(const brams_chem_args *a, int nthreads, brams_chem_stats *st,
+                          unsigned char *mask_out, const int32_t *blocks, int nblocks);
+/* ROS2 on the linear problem y' = A y, as ora_rodas3_linear. */
+int ora_ros2_linear(int n, const double *A, const double *y0, double h, int nsteps,
+                    double *y_out, double *err_out);
+/* ROS2's controller: dt * min(6, max(0.2, 0.9 err^-1/2)) in [dt_min, dt_max]. */
+double ora_ros2_dt_new(double dt, double max_err, double dt_min, double dt_max);
+
 /* Per-block counts of the last step: attempts, accepted sub-steps. */
 int ora_block_counts(int64_t *attempts, int64_t *substeps, int max_nob);
 /* Per-block largest error accepted at dt_min (0: none) of the last step. */

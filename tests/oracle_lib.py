@@ -72,6 +72,8 @@ def oracle():
         lib.ora_alloc_index.argtypes = [C.c_int] * 6 + [_ip, C.c_int]
         lib.ora_rodas3_dyndt.argtypes = [C.POINTER(ChemArgs), C.c_int, C.POINTER(ChemStats),
                                          C.POINTER(C.c_uint8)]
+        lib.ora_ros2_dyndt.argtypes = [C.POINTER(ChemArgs), C.c_int, C.POINTER(ChemStats),
+                                       C.POINTER(C.c_uint8)]
         lib.ora_block_counts.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int]
         lib.ora_use_external_sparse.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _oracle = lib
@@ -256,34 +258,35 @@ class HostStep:
 
 
 def run_oracle(wl, atol, rtol, nthreads=0, split_method=0, n_dyn_chem=1, sc_t=None, sc_t_dyn=None,
-               reference_sparse=False):
-    """Run the oracle step on a workload; returns (HostStep, stats, mask, rc)."""
+               reference_sparse=False, solver="rodas3"):
+    """Run the oracle step on a workload; returns (HostStep, stats, mask, rc).
+    solver: "rodas3" (chem_rodas3_dyndt) or "ros2" (chem_ros_dyndt)."""
     hs = HostStep(wl, atol, rtol, split_method, n_dyn_chem, sc_t, sc_t_dyn)
     st = ChemStats()
     mask = np.zeros(NS * NS, np.uint8)
     lib = oracle()
+    fn = {"rodas3": lib.ora_rodas3_dyndt, "ros2": lib.ora_ros2_dyndt}[solver]
     if reference_sparse:
         use_reference_sparse(True)
     try:
-        rc = lib.ora_rodas3_dyndt(C.byref(hs.args), nthreads, C.byref(st), ptr(mask, C.POINTER(C.c_uint8)))
+        rc = fn(C.byref(hs.args), nthreads, C.byref(st), ptr(mask, C.POINTER(C.c_uint8)))
     finally:
         if reference_sparse:
             use_reference_sparse(False)
     return hs, st, mask.reshape(NS, NS), rc
 
 
-def run_oracle_blocks(wl, atol, rtol, blocks, nthreads=0):
+def run_oracle_blocks(wl, atol, rtol, blocks, nthreads=0, solver="rodas3"):
     """The oracle step on block 1 plus the listed blocks (0-based); the other
     blocks' cells are left as they were.  Returns (HostStep, stats, mask, rc)."""
     hs = HostStep(wl, atol, rtol)
     st = ChemStats()
     mask = np.zeros(NS * NS, np.uint8)
     lib = oracle()
-    lib.ora_rodas3_dyndt_blocks.argtypes = [C.POINTER(ChemArgs), C.c_int, C.POINTER(ChemStats),
-                                            C.POINTER(C.c_uint8), _ip, C.c_int]
+    fn = {"rodas3": lib.ora_rodas3_dyndt_blocks, "ros2": lib.ora_ros2_dyndt_blocks}[solver]
+    fn.argtypes = [C.POINTER(ChemArgs), C.c_int, C.POINTER(ChemStats), C.POINTER(C.c_uint8), _ip, C.c_int]
     ids = np.ascontiguousarray(blocks, dtype=np.int32)
-    rc = lib.ora_rodas3_dyndt_blocks(C.byref(hs.args), nthreads, C.byref(st),
-                                     ptr(mask, C.POINTER(C.c_uint8)), ptr(ids, _ip), len(ids))
+    rc = fn(C.byref(hs.args), nthreads, C.byref(st), ptr(mask, C.POINTER(C.c_uint8)), ptr(ids, _ip), len(ids))
     return hs, st, mask.reshape(NS, NS), rc
 
 
@@ -398,3 +401,20 @@ def dt_new(dt, err, dt_min, dt_max):
     lib.ora_dt_new.argtypes = [C.c_double] * 4
     lib.ora_dt_new.restype = C.c_double
     return lib.ora_dt_new(dt, err, dt_min, dt_max)
+
+
+def ros2_linear(A, y0, h, nsteps):
+    lib = oracle()
+    lib.ora_ros2_linear.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _dp, _dp]
+    A = np.ascontiguousarray(A, np.float64); y0 = np.ascontiguousarray(y0, np.float64)
+    out = np.zeros(len(y0)); err = np.zeros(1)
+    rc = lib.ora_ros2_linear(len(y0), ptr(A), ptr(y0), h, nsteps, ptr(out), ptr(err))
+    assert rc == 0
+    return out, float(err[0])
+
+
+def ros2_dt_new(dt, err, dt_min, dt_max):
+    lib = oracle()
+    lib.ora_ros2_dt_new.argtypes = [C.c_double] * 4
+    lib.ora_ros2_dt_new.restype = C.c_double
+    return lib.ora_ros2_dt_new(dt, err, dt_min, dt_max)
