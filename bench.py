@@ -17,6 +17,8 @@ is also reported on its own).  From that warm state:
 Inputs stay resident in HBM; each leg A step also copies the state to a
 snapshot buffer for the accuracy check (a 322 MB device copy, ~0.1 ms).
 --path fp64 times leg B alone as the headline (no surrogate model needed).
+--path ros2 times the ROS2 solver (CHEMISTRY=3) the same way, from its own
+spin-up; --path nested the three nested grids of one model step.
 
 roofline: the kernel with the largest share of leg A's time, against its
 peak (FP64 for the RODAS3 fallback kernel, BF16 MFMA for the fused MLP);
@@ -62,38 +64,55 @@ def flop_model():
     return f_call, f_sub, f_att
 
 
+def ros2_flop_model():
+    """Algorithmic FP64 FLOP of the ROS2 step (chem_spack_ros_dyndt.f90), per
+    cell: per attempt the diagonal update, the elimination, 2 solves, f(y+k1/g)
+    and the stage vectors (y+k1/g 2n, rhs2 3n, y_new 4n, error term 4n, tol
+    3n, err 3n) with the norm; per accepted sub-step and per call as RODAS3."""
+    with open(os.path.join(ROOT, "brams_amd", "csrc", "generated", "relacs_tuv_flops.json")) as fh:
+        f = json.load(fh)
+    ns = 47
+    vec = (2 + 3 + 4 + 4 + 3 + 3) * ns + 2
+    f_att = ns + f["elimination"] + 2 * f["solve"] + f["fun"] + vec
+    f_call, f_sub, _ = flop_model()
+    return f_call, f_sub, f_att
+
+
 MLP_FLOP_PER_CELL = 3 * 2 * (68 * 256 + 3 * 256 * 256 + 256 * 47)
 BF16_PEAK_TFLOPS = 2500.0
 # HBM traffic measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by
 # the counters' reading of a 1 GiB device copy: x2.0 and x1.0) on this
-# workload, profiles/rodas3_hbm_traffic_pmc.md.  Per cell-attempt (read +
+# workload, profiles/rodas3_hbm_traffic_pmc.md (RODAS3) and
+# profiles/ros2_kernel_stats_and_traffic.md (ROS2).  Per cell-attempt (read +
 # written bytes), the same in every launch of a kernel: the spill traffic of
 # the per-lane LU and stage vectors, which depends on the register budget
 # (waves per SIMD).  The fused MLP: bytes per cell of one launch.  A kernel
 # without a measurement reports traffic null.
-RODAS3_BYTES_PER_CELL_ATTEMPT = {
+BYTES_PER_CELL_ATTEMPT = {
     "rodas3_blocks<64,false,false,4>": 49000 + 14200,
     "rodas3_blocks<64,false,false,2>": 39200 + 10800,
     "rodas3_blocks<64,false,false,1>": 29700 + 8300,
     "rodas3_blocks<64,false,true,2>": 39000 + 11100,
     "rodas3_blocks<64,false,true,1>": 29300 + 8500,
+    "ros2_blocks<64,4>": 31300 + 9350,          # profiles/ros2_kernel_stats_and_traffic.md
 }
 MLP_BYTES_PER_CELL = 1825
 FORM_WPS = {3: 1, 4: 2, 5: 4}
 
 
 def spill_traffic(kernel, cell_attempts):
-    b = RODAS3_BYTES_PER_CELL_ATTEMPT.get(kernel)
+    b = BYTES_PER_CELL_ATTEMPT.get(kernel)
     return None if b is None else b * cell_attempts
 
 
-def cpu_baseline(warm_wl, cold_wl, atol, rtol, warm_every=1, cold_every=4):
+def cpu_baseline(warm_wl, cold_wl, atol, rtol, warm_every=1, cold_every=4, solver="rodas3"):
     """The oracle (C restatement of the reference step, gcc -O2, pthreads over
     blocks) on every warm_every-th block of the warm state (every block: the
     whole warm step), with each block's carried last_accepted_dt (the
     reference's semantics for those blocks), and on every cold_every-th block
     of the seeded cold state; block 1 runs first in both, as it fixes the
-    mask."""
+    mask (RODAS3).  solver "ros2": chem_ros_dyndt's restatement, no
+    calibration run."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import oracle_lib
@@ -104,15 +123,16 @@ def cpu_baseline(warm_wl, cold_wl, atol, rtol, warm_every=1, cold_every=4):
         blocks = np.arange(max(every, 1), nob, every, dtype=np.int32)
         cells = int(wl["block_end"][0]) + int(np.sum(np.asarray(wl["block_end"])[blocks]))
         t0 = time.perf_counter()
-        hs, st, mask, rc = oracle_lib.run_oracle_blocks(wl, atol, rtol, blocks, nthreads=cores)
+        hs, st, mask, rc = oracle_lib.run_oracle_blocks(wl, atol, rtol, blocks, nthreads=cores, solver=solver)
         dt = time.perf_counter() - t0
         out[tag] = {"value": cells / dt, "cells": cells, "blocks": len(blocks) + 1, "wall_s": dt,
                     "cell_attempts": int(st.cell_attempts)}
     w, c = out["warm"], out["cold"]
-    calib = calibrate_oracle(warm_wl, atol, rtol, cores)
+    calib = calibrate_oracle(warm_wl, atol, rtol, cores) if solver == "rodas3" else None
+    name = {"rodas3": "chem_rodas3_dyndt", "ros2": "chem_ros_dyndt"}[solver]
     return {"value": w["value"], "unit": "cell-steps/s", "cores": cores, "kind": "port",
             "calibration": calib,
-            "sample": "oracle (C restatement of chem_rodas3_dyndt with the reference's kernels' "
+            "sample": "oracle (C restatement of " + name + " with the reference's kernels' "
                       "arithmetic, gcc -O2 -ffp-contract=off, pthreads over blocks) on %s of the warm state "
                       "(%d blocks, %d cells x 1 step, carried last_accepted_dt, %.1f s wall, "
                       "%d cell-attempts)" % (_which(warm_every), w["blocks"], w["cells"], w["wall_s"],
@@ -232,6 +252,105 @@ def bench_nested(args):
     print(json.dumps(line), flush=True)
 
 
+def calibration_copy(mib, dev, log):
+    """A device-to-device copy of known size (mib MiB read and written), the
+    reference reading that scales rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    if not mib:
+        return
+    import torch
+    src = torch.ones(mib << 20, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("calibration copy: %d MiB read + %d MiB written" % (mib, mib))
+    del src, dst
+
+
+def bench_ros2(args):
+    """The plain FP64 ROS2 solver (CHEMISTRY=3, brams_chem_ros_dyndt_device)
+    on the headline workload: --spinup ROS2 steps from the seeded state (the
+    first is the cold step), then --steps timed steps, bracketed by
+    synchronisations.  value: cell-steps/s of the timed steps."""
+    import numpy as np
+    import torch
+    from brams_amd import abi
+    from brams_amd.chem import DeviceStep
+    from brams_amd.workload import Grid, jphoto_from_sza, make_workload, tolerances
+
+    def log(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
+
+    grid = Grid(args.nx, args.ny, args.nz, dtlt=120.0)
+    wl = make_workload(grid, seed=0, sza="split", plume_sigma=20.0, plume_factor=50.0)
+    atol, rtol = tolerances(args.tol)
+    K, S = args.steps, args.spinup
+    ds = DeviceStep(wl, atol, rtol, device=0, solver="ros2")
+    jph = [torch.from_numpy(np.ascontiguousarray(jphoto_from_sza(grid, wl["sza_col"] + 0.5 * t)
+                                                 .ravel(order="F"))).to(ds.dev) for t in range(S + K)]
+    log("ros2: workload %dx%dx%d = %d cells, %d blocks" % (args.nx, args.ny, args.nz, grid.ncell,
+                                                           len(wl["block_end"])))
+    calibration_copy(args.copy_calib_mb, ds.dev, log)
+    cold = None
+    for t in range(S):
+        ds.args.jphoto = jph[t].data_ptr()
+        ds.step()
+        st = abi.stats()
+        if t == 0:
+            cold = {"kernel_ms": st.kernel_ms, "attempts_per_cell": st.cell_attempts / st.cells,
+                    "substeps_per_cell": st.cell_substeps / st.cells,
+                    "cell_steps_per_s": grid.ncell / (st.kernel_ms * 1e-3)}
+        log("ros2 spin-up step %d: %.1f ms (%d cell-attempts)" % (t, st.kernel_ms, st.cell_attempts))
+    warm_sc = ds.sc_p.clone()
+    warm_dt = ds.last_dt.clone()
+    f_call, f_sub, f_att = ros2_flop_model()
+    wall, main_ms, flops_main, att_main = 0.0, 0.0, 0.0, 0.0
+    for t in range(K):
+        ds.args.jphoto = jph[S + t].data_ptr()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ds.step()
+        torch.cuda.synchronize()
+        wall += time.perf_counter() - t0
+        st = abi.stats()
+        main_ms += st.main_ms
+        share = st.main_cells / st.cells
+        flops_main += (f_call * st.cells + f_sub * st.cell_substeps + f_att * st.cell_attempts) * share
+        att_main += st.cell_attempts * share
+        log("ros2 step %d: %.1f ms (main kernel %.1f ms, %d cell-attempts)"
+            % (t, st.kernel_ms, st.main_ms, st.cell_attempts))
+    n64 = sum(1 for b in wl["block_end"] if b <= 64)
+    kernel = "ros2_blocks<64,%d>" % (4 if n64 >= 4096 else 1)      # ros2_launch.hip's launch shape
+    achieved = flops_main / (main_ms * 1e-3) / 1e12
+    line = {
+        "metric": "chem cell-steps/s, FP64 ROS2 (CHEMISTRY=3) warm rollout",
+        "value": grid.ncell * K / wall, "unit": "cell-steps/s", "n_gpus": 1, "steps": K,
+        "warmup": S, "ms_per_step": 1e3 * wall / K, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Philox generator, brams_amd/workload.py)",
+        "config": {"workload": "RELACS_TUV chemistry rollout, %dx%dx%d cells, %d x 120 s ROS2 steps after "
+                               "%d ROS2 spin-up steps, SZA split day/night +0.5 deg/step, NOx/VOC plume "
+                               "sigma 20 x50, seed 0" % (args.nx, args.ny, args.nz, K, S),
+                   "path": "plain FP64 ROS2 (GPU), warm rollout", "tolerances": args.tol,
+                   "cells": grid.ncell, "blocks": len(wl["block_end"]), "fp64_cold_step": cold,
+                   "flop_model_ros2": {"per_cell_call": f_call, "per_substep": f_sub, "per_attempt": f_att}},
+        "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": spill_traffic(kernel, att_main / K),
+                     "kernel": kernel,
+                     "avg_ms": main_ms / K, "algorithmic_bytes": 528 * grid.ncell},
+    }
+    if not args.no_cpu:
+        warm_wl = dict(wl)
+        ds.sc_p.copy_(warm_sc)
+        warm_wl["sc_p"] = ds.species_host()
+        warm_wl["jphoto"] = jphoto_from_sza(grid, wl["sza_col"] + 0.5 * S)
+        warm_wl["last_accepted_dt"] = warm_dt.cpu().numpy().copy()
+        log("cpu baseline: ROS2 oracle on block samples of the warm and the cold state")
+        line["cpu_baseline"] = cpu_baseline(warm_wl, wl, atol, rtol, warm_every=4, cold_every=16,
+                                            solver="ros2")
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,9 +361,10 @@ def main():
     ap.add_argument("--ny", type=int, default=200)
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--tol", default="baseline", choices=["baseline", "reference"])
-    ap.add_argument("--path", default="surrogate", choices=["surrogate", "fp64", "nested"],
+    ap.add_argument("--path", default="surrogate", choices=["surrogate", "fp64", "nested", "ros2"],
                     help="nested: the plain FP64 step on three nested grids (200x200x40 at dt 120 s, "
-                         "98x98x40 at 60 s, 50x50x40 at 30 s), one call per grid and in one pass")
+                         "98x98x40 at 60 s, 50x50x40 at 30 s), one call per grid and in one pass; "
+                         "ros2: the ROS2 solver (CHEMISTRY=3) rollout on the 200x200x40 workload")
     ap.add_argument("--model", default=os.path.join(ROOT, "brams_amd", "models", "relacs_tuv_surrogate.npz"))
     ap.add_argument("--spread-tol", type=float, default=None, help="override the model's spread check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -259,6 +379,8 @@ def main():
 
     if args.path == "nested":
         return bench_nested(args)
+    if args.path == "ros2":
+        return bench_ros2(args)
 
     import numpy as np
     import torch
@@ -282,14 +404,7 @@ def main():
     fbuf = torch.empty((grid.ncell, NIN), dtype=torch.float32, device=dev)
     valid = torch.ones(grid.ncell, dtype=torch.bool, device=dev)
     log("workload %dx%dx%d = %d cells, %d blocks" % (args.nx, args.ny, args.nz, grid.ncell, len(wl["block_end"])))
-    if args.copy_calib_mb:
-        src = torch.ones(args.copy_calib_mb << 20, dtype=torch.uint8, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("calibration copy: %d MiB read + %d MiB written" % (args.copy_calib_mb, args.copy_calib_mb))
-        del src, dst
+    calibration_copy(args.copy_calib_mb, dev, log)
 
     def block_spread(tag):
         att, _, _, _ = abi.block_status(len(wl["block_end"]))

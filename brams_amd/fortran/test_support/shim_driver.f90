@@ -1,11 +1,13 @@
-! Test driver of the Fortran shim: reads one chemistry call from a stream
+! Test driver of the Fortran drop-ins: reads one chemistry call from a stream
 ! file written by tests/test_gpu_fortran_shim.py, calls chem_rodas3_dyndt
-! exactly as chemistry.f90:443-453 does, and writes the updated state.
+! exactly as chemistry.f90:443-453 does (or, with a third argument "ros2",
+! chem_ros_dyndt as chemistry.f90:430-440 does), and writes the updated state.
 PROGRAM shim_driver
   USE mem_chem1, ONLY: chem1_vars
   USE mem_spack, ONLY: spack_type, spack_type_2d
   USE Extras, ONLY: ext3d
   USE mod_chem_spack_rodas3_dyndt, ONLY: chem_rodas3_dyndt
+  USE mod_chem_spack_ros_dyndt, ONLY: chem_ros_dyndt
   IMPLICIT NONE
   INTEGER :: m1, m2, m3, nob, mbs, nsp, nr, nrp, ntr, nnt, ndyn, n, u
   REAL :: dtlt, cp, cpor, p00
@@ -18,12 +20,14 @@ PROGRAM shim_driver
   TYPE(spack_type_2d) :: spack_2d(1,1)
   TYPE(ext3d) :: extra3d(1)
   LOGICAL :: gnz
-  CHARACTER(LEN=256) :: fin, fout
+  CHARACTER(LEN=256) :: fin, fout, solver
   CHARACTER(LEN=10) :: photoj
   CHARACTER(LEN=20) :: split
 
   CALL get_command_argument(1, fin)
   CALL get_command_argument(2, fout)
+  solver = 'rodas3'
+  IF (command_argument_count() >= 3) CALL get_command_argument(3, solver)
   OPEN(newunit=u, file=TRIM(fin), access='stream', form='unformatted', status='old')
   READ(u) m1, m2, m3, nob, mbs, nsp, nr, nrp, ntr, nnt, ndyn
   READ(u) dtlt, cp, cpor, p00
@@ -44,10 +48,17 @@ PROGRAM shim_driver
   photoj = 'FAST-TUV'
   split = 'SYMMETRIC'
   gnz = .FALSE.
-  CALL chem_rodas3_dyndt(m1,m2,m3,dtlt,pp,pi0,theta,rv,dn0,cosz,rcp, &
+  IF (TRIM(solver) == 'ros2') THEN
+     CALL chem_ros_dyndt(m1,m2,m3,dtlt,pp,pi0,theta,rv,dn0,cosz,rcp, &
                          nsp,nr,nrp,weight,photoj,200,ntr,nnt,tidx,nidx,chem1_g,nob,mbs, &
                          block_end,ik,ii,ij,kij,last_dt,spack,spack_2d,atol,rtol,gnz, &
-                         ndyn,split,cp,cpor,p00,1,extra3d,jphoto,att)
+                         ndyn,split,cp,cpor,p00,jphoto,att)
+  ELSE
+     CALL chem_rodas3_dyndt(m1,m2,m3,dtlt,pp,pi0,theta,rv,dn0,cosz,rcp, &
+                            nsp,nr,nrp,weight,photoj,200,ntr,nnt,tidx,nidx,chem1_g,nob,mbs, &
+                            block_end,ik,ii,ij,kij,last_dt,spack,spack_2d,atol,rtol,gnz, &
+                            ndyn,split,cp,cpor,p00,1,extra3d,jphoto,att)
+  END IF
   OPEN(newunit=u, file=TRIM(fout), access='stream', form='unformatted', status='replace')
   DO n = 1, nsp
      WRITE(u) chem1_g(n)%sc_p

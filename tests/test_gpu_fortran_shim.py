@@ -1,5 +1,6 @@
-"""The Fortran drop-in (brams_amd/fortran/mod_chem_spack_rodas3_dyndt.f90)
-called like chemistry.f90:443-453 from a Fortran driver, through
+"""The Fortran drop-ins (brams_amd/fortran/mod_chem_spack_rodas3_dyndt.f90,
+mod_chem_spack_ros_dyndt.f90) called like chemistry.f90:443-453 / 430-440
+from a Fortran driver, through
 ISO_C_BINDING into the HIP library: same result as the C-ABI called from
 Python, and parity with the oracle."""
 import os
@@ -56,15 +57,18 @@ def _write_input(path, wl, atol, rtol):
             fh.write(np.asfortranarray(x).tobytes(order="F"))
 
 
-def test_fortran_shim_matches_c_abi_and_oracle(tmp_path):
+@pytest.mark.parametrize("solver", ["rodas3", "ros2"])
+def test_fortran_shim_matches_c_abi_and_oracle(tmp_path, solver):
+    """chem_rodas3_dyndt (mod_chem_spack_rodas3_dyndt) and chem_ros_dyndt
+    (mod_chem_spack_ros_dyndt) with the reference's argument lists."""
     assert os.path.exists(DRIVER), "Fortran driver not built"
     grid = Grid(10, 8, 9)
     wl = make_workload(grid, seed=11)
     atol, rtol = tolerances("baseline")
     fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
     _write_input(fin, wl, atol, rtol)
-    print("case fortran shim 10x8x9 seed=11", flush=True)
-    r = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
+    print("case fortran shim %s 10x8x9 seed=11" % solver, flush=True)
+    r = subprocess.run([DRIVER, fin, fout, solver], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     n3 = grid.m1 * grid.m2 * grid.m3
     raw = open(fout, "rb").read()
@@ -72,11 +76,11 @@ def test_fortran_shim_matches_c_abi_and_oracle(tmp_path):
     fort = [np.asfortranarray(sc[s].reshape(grid.shape, order="F")) for s in range(47)]
     fort_dt = np.frombuffer(raw[47 * n3 * 4:], np.float64)
     from brams_amd.chem import run_workload_host
-    py, py_dt, rc, _ = run_workload_host(wl, atol, rtol)
+    py, py_dt, rc, _ = run_workload_host(wl, atol, rtol, solver=solver)
     assert rc == 0
     for a, b in zip(fort, py):
         assert np.array_equal(a, b)
     assert np.array_equal(fort_dt, py_dt)
-    hs, st, mask, orc = oracle_lib.run_oracle(wl, atol, rtol)
+    hs, st, mask, orc = oracle_lib.run_oracle(wl, atol, rtol, solver=solver)
     rep = parity_report(fort, hs.sc_p, grid, wl)
     assert rep["nfail"] == 0, rep
