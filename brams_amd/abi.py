@@ -21,6 +21,7 @@ _ip = C.POINTER(C.c_int32)
 EXPORTS = ["brams_chem_init", "brams_chem_rodas3_dyndt", "brams_chem_rodas3_dyndt_device",
            "brams_chem_rodas3_dyndt_device_noalloc", "brams_chem_rodas3_dyndt_device_grids",
            "brams_chem_accum_device", "brams_chem_ros_dyndt", "brams_chem_ros_dyndt_device",
+           "brams_chem_qssa", "brams_chem_qssa_device",
            "brams_chem_alloc_index", "brams_chem_get_stats", "brams_chem_last_error",
            "brams_chem_finalize", "brams_chem_block_status", "brams_chem_block_thresholds",
            "brams_chem_set_form", "brams_chem_get_tuning"]
@@ -109,6 +110,8 @@ def lib():
         L.brams_chem_rodas3_dyndt_device.argtypes = [C.POINTER(ChemDevArgs)]
         L.brams_chem_ros_dyndt.argtypes = [C.POINTER(ChemArgs)]
         L.brams_chem_ros_dyndt_device.argtypes = [C.POINTER(ChemDevArgs)]
+        L.brams_chem_qssa.argtypes = [C.POINTER(ChemArgs), C.c_int32]
+        L.brams_chem_qssa_device.argtypes = [C.POINTER(ChemDevArgs), C.c_int32]
         L.brams_chem_alloc_index.argtypes = [C.c_int32] * 6 + [_ip, C.c_int32]
         L.brams_chem_get_stats.argtypes = [C.POINTER(ChemStats)]
         L.brams_chem_last_error.restype = C.c_char_p

@@ -33,13 +33,14 @@ OBJ = os.path.join(PKG, "_build")
 # (rodas3_launch*.hip, rodas3_grids.hip, ros2_launch.hip) that compile in
 # parallel; the slab kernels (two minutes) live in rodas3_slab.hip.
 HIP_UNITS = ["rodas3_launch.hip", "rodas3_launch_b.hip", "rodas3_grids.hip", "ros2_launch.hip",
-             "rodas3_slab.hip", "brams_chem.hip", "surrogate.hip"]
+             "rodas3_slab.hip", "qssa_launch.hip", "brams_chem.hip", "surrogate.hip"]
 _RODAS3_DEPS = ["rodas3_kernel.h", "rodas3_launch.h", "mech_families.h", "generated/relacs_tuv_gen.h"]
 UNIT_DEPS = {
     "rodas3_launch.hip": _RODAS3_DEPS,
     "rodas3_launch_b.hip": _RODAS3_DEPS,
     "rodas3_grids.hip": _RODAS3_DEPS,
     "ros2_launch.hip": _RODAS3_DEPS,
+    "qssa_launch.hip": _RODAS3_DEPS,
     "rodas3_slab.hip": ["rodas3_slab.h", "rodas3_slab_launch.h", "rodas3_kernel.h", "mech_families.h",
                         "generated/relacs_tuv_gen.h", "generated/relacs_tuv_slab.h"],
 }

@@ -5,9 +5,10 @@ mod_chem_spack_rodas3_dyndt::chem_rodas3_dyndt
 (src/brams/ccatt/chem_spack_rodas3_dyndt.f90:41-131) for the arguments the
 RELACS_TUV path reads; it updates chem1_sc_p (or sc_t) and last_accepted_dt in
 place like the Fortran routine.  chem_ros_dyndt() is the same for the ROS2
-solver (chem_spack_ros_dyndt.f90:39-45, CHEMISTRY=3).  DeviceStep runs either
-step on arrays that already live in HBM (torch tensors are used only as
-device memory).
+solver (chem_spack_ros_dyndt.f90:39-45, CHEMISTRY=3), and chem_qssa() for
+the QSSA solver (chem_spack_qssa.f90:37-43, CHEMISTRY=1), which adds every
+species' tendency to sc_t.  DeviceStep runs any of them on arrays that
+already live in HBM (torch tensors are used only as device memory).
 """
 import ctypes as C
 
@@ -100,6 +101,49 @@ def chem_ros_dyndt(*args, **kw):
     return chem_rodas3_dyndt(*args, solver="ros2", **kw)
 
 
+def chem_qssa(m1, m2, m3, ia, iz, ja, jz, itchim, dtlt, pp, pi0, theta, rv, jphoto, nspecies, nr,
+              nr_photo, weight, PhotojMethod, chem1_sc_p, chem1_sc_t, cp, cpor, p00, device=0):
+    """chem_qssa (chem_spack_qssa.f90:37-43) on one grid: every species of
+    chem1_sc_p (ppbm) is read, chem1_sc_t += (new - old)/dtlt in place for
+    every species; chem1_sc_p is not changed.  Returns the number of cells
+    where a species reached 1e20 (the reference stops there)."""
+    L = abi.lib()
+    abi.check(L.brams_chem_init(device), "brams_chem_init")
+    a = abi.ChemArgs()
+    a.m1, a.m2, a.m3, a.ia, a.iz, a.ja, a.jz = m1, m2, m3, ia, iz, ja, jz
+    a.dtlt = dtlt
+    a.n_dyn_chem = 1
+    a.photoj_method = abi.PHOTOJ[PhotojMethod.strip()]
+    a.cp, a.cpor, a.p00 = cp, cpor, p00
+    a.pp, a.pi0, a.theta, a.rv = _f32(pp), _f32(pi0), _f32(theta), _f32(rv)
+    jph = np.asarray(jphoto)
+    if jph.dtype != np.float64 or not jph.flags["F_CONTIGUOUS"]:
+        raise abi.ChemError("jphoto must be a Fortran-ordered float64 array")
+    a.jphoto = jph.ctypes.data_as(C.POINTER(C.c_double))
+    a.nspecies, a.nr, a.nr_photo = nspecies, nr, nr_photo
+    w = np.ascontiguousarray(weight, dtype=np.float32)
+    a.weight = w.ctypes.data_as(_fp)
+    tab_p = (_fp * nspecies)(*[_f32(x) for x in chem1_sc_p])
+    tab_t = (_fp * nspecies)(*[_f32(x) for x in chem1_sc_t])
+    a.sc_p = C.cast(tab_p, C.POINTER(_fp))
+    a.sc_t = C.cast(tab_t, C.POINTER(_fp))
+    return abi.check(L.brams_chem_qssa(C.byref(a), itchim), "brams_chem_qssa")
+
+
+def run_workload_qssa_host(wl, itchim=75, sc_t=None):
+    """Convenience: chem_qssa on a brams_amd.workload dict; returns the
+    tendencies (copies of sc_t, zeros by default) and the status."""
+    mech = load_mechanism()
+    g = wl["grid"]
+    sct = ([np.zeros_like(x, order="F") for x in wl["sc_p"]] if sc_t is None
+           else [np.array(x, copy=True, order="F") for x in sc_t])
+    rc = chem_qssa(g.m1, g.m2, g.m3, g.ia, g.iz, g.ja, g.jz, itchim, g.dtlt, wl["pp"], wl["pi0"],
+                   wl["theta"], wl["rv"], wl["jphoto"], mech.nspecies, mech.nr, 17,
+                   [s.weight for s in mech.species], "FAST-TUV", wl["sc_p"], sct, 1004.0,
+                   1004.0 / 287.0, 1.0e5)
+    return sct, rc
+
+
 def run_workload_host(wl, atol, rtol, split_method="SYMMETRIC", n_dyn_chem=1, sc_t=None,
                       sc_t_dyn=None, solver="rodas3"):
     """Convenience: run chem_rodas3_dyndt (or chem_ros_dyndt) on a
@@ -180,6 +224,22 @@ class DeviceStep:
         """Restore the initial state (for repeated timing of the same step)."""
         self.sc_p.copy_(self.sc_p0)
         self.last_dt.copy_(self.last_dt0)
+
+    def qssa(self, itchim=75):
+        """brams_chem_qssa_device on the resident state: adds every species'
+        tendency to self.sc_t ([47, n3d], zeros at first use); sc_p is not
+        changed.  Returns the number of cells that reached 1e20."""
+        if getattr(self, "sc_t", None) is None:
+            self.sc_t = self.torch.zeros_like(self.sc_p)
+            self.args.sc_t = self.sc_t.data_ptr()
+        L = abi.lib()
+        return abi.check(L.brams_chem_qssa_device(C.byref(self.args), itchim), "brams_chem_qssa_device")
+
+    def tendency_host(self):
+        """sc_t back on the host as a list of Fortran-ordered (m1,m2,m3) arrays."""
+        g = self.grid
+        arr = self.sc_t.cpu().numpy()
+        return [np.asfortranarray(arr[s].reshape((g.m1, g.m2, g.m3), order="F")) for s in range(arr.shape[0])]
 
     def step(self):
         return abi.check(getattr(abi.lib(), self.entry)(C.byref(self.args)), self.entry)

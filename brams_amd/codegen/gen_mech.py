@@ -124,6 +124,13 @@ def rate_expr(fam, p, r):
     raise ValueError(fam)
 
 
+def nflop_sum_pl(coefs):
+    """FP64 operations of a sum of terms c * x (c = 1 not multiplied)."""
+    if not coefs:
+        return 0
+    return (len(coefs) - 1) + sum(1 for c in coefs if abs(c) != 1.0)
+
+
 def main():
     m, ns, nr, react, prod = load()
     jt = jac_terms(ns, react, prod)
@@ -222,6 +229,33 @@ def main():
             w("  f[%d] = 0.0;" % s)
         else:
             w("  f[%d] = %s;" % (s, sum_terms([(c, "w%d" % r) for c, r in terms])))
+    w("}")
+    w("")
+
+    # ---------------- production / loss of the QSSA split
+    def dwexpr(r, s):
+        rs = react[r]
+        if len(rs) == 1:
+            return "rk[%d]" % r
+        o = rs[1] if rs[0] == s else rs[0]
+        return "(rk[%d] * y[%d])" % (r, o)
+
+    w("// QSSA's P - L c split: chem_spack_rates.f90:63-192, chem_spack_dratedc.f90:62-296,")
+    w("// chem_spack_fexprod.f90:64-448, chem_spack_fexloss.f90:64-284.  P: the positive")
+    w("// stoichiometric terms c w_r, L: -c dw_rs over the negative ones (dw_rs = rk_r y_other,")
+    w("// y_s itself in a self-reaction), in reaction order; rows with no term are 0.")
+    w("__device__ __forceinline__ void mech_prod_loss(const double* rk, const double* y, double* P, double* L) {")
+    upos = sorted({r for terms in prod.values() for c, r in terms if c > 0})
+    for r in upos:
+        w("  const double w%d = %s;" % (r, wexpr(r)))
+    fl_pl = sum(len(react[r]) for r in upos)
+    for s in range(ns):
+        pt = [(c, "w%d" % r) for c, r in prod.get(s, []) if c > 0]
+        lt = [(-c, dwexpr(r, s)) for c, r in prod.get(s, []) if c < 0]
+        w("  P[%d] = %s;" % (s, sum_terms(pt) if pt else "0.0"))
+        w("  L[%d] = %s;" % (s, sum_terms(lt) if lt else "0.0"))
+        fl_pl += nflop_sum_pl([c for c, _ in pt]) + nflop_sum_pl([c for c, _ in lt])
+        fl_pl += sum(1 for _, e in lt if "*" in e)
     w("}")
     w("")
 
@@ -359,7 +393,7 @@ def main():
                 ar_kin += 1
             elif q[0] == "mul_troe":
                 tr["pow"] += 3; tr["log10"] += 1; ar_kin += 15
-    flops = {"fun": fl_fun, "jacobian_assembly": fl_jac, "elimination": fl_elim,
+    flops = {"fun": fl_fun, "prod_loss": fl_pl, "jacobian_assembly": fl_jac, "elimination": fl_elim,
              "factor": fl_jac + fl_elim, "solve": fl_solve, "nlu": nlu, "nl": nl, "nu": nu,
              "kinetic_arith": ar_kin, "kinetic_transcendentals": tr,
              "note": "FP64 add/sub/mul/div counted once each (an FMA is 2), "

@@ -130,6 +130,8 @@ struct State {
   int force_form = -1;           // brams_chem_set_form (-1: autotune per grid)
   // autotuning scratch (copies of the state a step writes)
   DevBuf tune_sc, tune_dt;
+  // QSSA: count of cells that reached 1e20
+  DevBuf qssa_fatal;
   // several grids in one pass: per-grid kernel parameters and work lists
   DevBuf multi_ps, multi_work;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evb = nullptr, evm = nullptr;
@@ -538,17 +540,24 @@ int collect_stats(hipStream_t st) {
   return (int)s.failed_blocks;
 }
 
-int check_common(int m1, int m2, int m3, int ia, int iz, int ja, int jz, int ndyn, int split,
-                 int photoj, float dtlt) {
+// grid, photolysis method and time step: the checks every solver shares
+int check_grid(int m1, int m2, int m3, int ia, int iz, int ja, int jz, int photoj, float dtlt) {
   if (m1 < 3 || m2 < 1 || m3 < 1) return fail(-2, "bad grid extents %d %d %d", m1, m2, m3);
   if (ia < 1 || iz > m2 || ia > iz || ja < 1 || jz > m3 || ja > jz)
     return fail(-2, "bad interior bounds ia=%d iz=%d ja=%d jz=%d", ia, iz, ja, jz);
-  if (ndyn < 1) return fail(-2, "N_DYN_CHEM must be >= 1");
-  if (split < 0 || split > 2) return fail(-2, "unknown split_method %d", split);
   if (photoj == BRAMS_PHOTOJ_LUT)
     return fail(-4, "PhotojMethod LUT is not supported (RELACS_TUV uses FAST-TUV)");
   if (photoj < 0 || photoj > 2) return fail(-2, "unknown PhotojMethod %d", photoj);
   if (!(dtlt > 0.f)) return fail(-2, "dtlt must be positive");
+  return 0;
+}
+
+int check_common(int m1, int m2, int m3, int ia, int iz, int ja, int jz, int ndyn, int split,
+                 int photoj, float dtlt) {
+  int rc = check_grid(m1, m2, m3, ia, iz, ja, jz, photoj, dtlt);
+  if (rc) return rc;
+  if (ndyn < 1) return fail(-2, "N_DYN_CHEM must be >= 1");
+  if (split < 0 || split > 2) return fail(-2, "unknown split_method %d", split);
   return 0;
 }
 
@@ -1208,9 +1217,110 @@ int host_step(const brams_chem_args* a, int solver) {
   return collect_stats(st);
 }
 
+// One QSSA step (chem_qssa) on device arrays already in p: the counter is
+// cleared, the kernel runs between the step events, and the call waits for
+// it to read the count of cells that reached 1e20 (the stats' failed count).
+int qssa_step(const brams::StepParams& p, int itchim, long long ncell, hipStream_t st) {
+  HIP_TRY(g.qssa_fatal.ensure(sizeof(int)));
+  HIP_TRY(hipMemsetAsync(g.qssa_fatal.p, 0, sizeof(int), st));
+  HIP_TRY(hipEventRecord(g.ev0, st));
+  HIP_TRY(brams::launch_qssa(p, itchim, ncell, g.qssa_fatal.as<int>(), st));
+  HIP_TRY(hipEventRecord(g.ev1, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int nfatal = 0;
+  HIP_TRY(hipMemcpy(&nfatal, g.qssa_fatal.p, sizeof(int), hipMemcpyDeviceToHost));
+  brams_chem_stats s{};
+  s.cells = ncell;
+  s.failed_blocks = nfatal;
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+  s.kernel_ms = ms;
+  s.main_ms = ms;
+  s.main_cells = ncell;
+  s.kernel_form = -1;
+  g.stats = s;
+  return nfatal;
+}
+
+int check_qssa(int itchim, int nspecies) {
+  if (itchim < 1) return fail(-2, "itchim must be >= 1 (chemistry.f90 passes 75)");
+  if (nspecies != MECH_NS) return fail(-2, "nspecies %d differs from RELACS_TUV %d", nspecies, MECH_NS);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int brams_chem_qssa_device(const brams_chem_dev_args* a, int32_t itchim) {
+  if (!a) return fail(-2, "null args");
+  int rc = check_grid(a->m1, a->m2, a->m3, a->ia, a->iz, a->ja, a->jz, a->photoj_method, a->dtlt);
+  if (rc || (rc = check_qssa(itchim, MECH_NS))) return rc;
+  if (!a->pp || !a->pi0 || !a->theta || !a->rv || !a->jphoto || !a->weight || !a->sc_p || !a->sc_t)
+    return fail(-2, "null device pointer (QSSA needs pp, pi0, theta, rv, jphoto, weight, sc_p, sc_t)");
+  if (!g.init && (rc = brams_chem_init(0))) return rc;
+  brams::StepParams p;
+  fill_params(a, p);
+  const long long ncell = (long long)(a->m1 - 2) * (a->iz - a->ia + 1) * (a->jz - a->ja + 1);
+  return qssa_step(p, itchim, ncell, (hipStream_t)a->stream);
+}
+
+int brams_chem_qssa(const brams_chem_args* a, int32_t itchim) {
+  if (!a) return fail(-2, "null args");
+  int rc = check_grid(a->m1, a->m2, a->m3, a->ia, a->iz, a->ja, a->jz, a->photoj_method, a->dtlt);
+  if (rc || (rc = check_qssa(itchim, a->nspecies))) return rc;
+  if (a->nr != MECH_NR || a->nr_photo != MECH_NPHOT)
+    return fail(-2, "mechanism sizes %d/%d differ from RELACS_TUV %d/%d", a->nr, a->nr_photo, MECH_NR,
+                MECH_NPHOT);
+  if (!a->pp || !a->pi0 || !a->theta || !a->rv || !a->jphoto || !a->weight || !a->sc_p || !a->sc_t)
+    return fail(-2, "null argument (QSSA needs pp, pi0, theta, rv, jphoto, weight, sc_p, sc_t)");
+  for (int s = 0; s < MECH_NS; ++s)
+    if (!a->sc_p[s] || !a->sc_t[s]) return fail(-2, "species %d has no sc_p or sc_t array", s + 1);
+  if (!g.init && (rc = brams_chem_init(0))) return rc;
+  const long long n3d = (long long)a->m1 * a->m2 * a->m3;
+  const size_t fsz = sizeof(float) * n3d;
+  HIP_TRY(g.sc_p.ensure(fsz * MECH_NS));
+  HIP_TRY(g.sc_t.ensure(fsz * MECH_NS));
+  HIP_TRY(g.met.ensure(fsz * 4));
+  HIP_TRY(g.jphoto.ensure(sizeof(double) * MECH_NPHOT * n3d));
+  HIP_TRY(g.consts.ensure(1536));
+  hipStream_t st = nullptr;
+  char* sc = g.sc_p.as<char>();
+  char* tc = g.sc_t.as<char>();
+  for (int s = 0; s < MECH_NS; ++s) {
+    HIP_TRY(hipMemcpyAsync(sc + s * fsz, a->sc_p[s], fsz, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(tc + s * fsz, a->sc_t[s], fsz, hipMemcpyHostToDevice, st));
+  }
+  char* met = g.met.as<char>();
+  HIP_TRY(hipMemcpyAsync(met + 0 * fsz, a->pp, fsz, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(met + 1 * fsz, a->pi0, fsz, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(met + 2 * fsz, a->theta, fsz, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(met + 3 * fsz, a->rv, fsz, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(g.jphoto.p, a->jphoto, sizeof(double) * MECH_NPHOT * n3d, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpy(g.consts.p, a->weight, sizeof(float) * MECH_NS, hipMemcpyHostToDevice));
+  brams::StepParams p{};
+  p.m1 = a->m1; p.m2 = a->m2; p.m3 = a->m3;
+  p.ia = a->ia; p.iz = a->iz; p.ja = a->ja; p.jz = a->jz;
+  p.nk = a->m1 - 2; p.ni = a->iz - a->ia + 1;
+  p.dtlt = a->dtlt;
+  p.cp = a->cp; p.cpor = a->cpor; p.p00 = a->p00;
+  p.pp = (const float*)(met + 0 * fsz);
+  p.pi0 = (const float*)(met + 1 * fsz);
+  p.theta = (const float*)(met + 2 * fsz);
+  p.rv = (const float*)(met + 3 * fsz);
+  p.jphoto = g.jphoto.as<double>();
+  p.weight = g.consts.as<float>();
+  p.sc_p = g.sc_p.as<float>();
+  p.sc_t = g.sc_t.as<float>();
+  p.n3d = n3d;
+  const long long ncell = (long long)(a->m1 - 2) * (a->iz - a->ia + 1) * (a->jz - a->ja + 1);
+  const int nfatal = qssa_step(p, itchim, ncell, st);
+  if (nfatal < 0) return nfatal;
+  for (int s = 0; s < MECH_NS; ++s)
+    HIP_TRY(hipMemcpyAsync(a->sc_t[s], tc + s * fsz, fsz, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return nfatal;
+}
 
 int brams_chem_block_status(int32_t* attempts, int32_t* substeps, int32_t* failed,
                             float* forced_err, int32_t max_nob) {

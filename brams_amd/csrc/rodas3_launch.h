@@ -26,6 +26,11 @@ hipError_t launch_rodas3_64w2(unsigned nblocks, const StepParams& p, int block0,
 // from kRodas3ManyBlocks blocks on, else one.
 hipError_t launch_ros2(int wg, unsigned nblocks, const StepParams& p, int block0, hipStream_t st);
 
+// The QSSA solver (qssa_launch.hip): one lane per interior cell, itchim
+// sub-steps; nfatal (device int, zeroed by the caller) counts the cells where
+// a species reached 1e20.
+hipError_t launch_qssa(const StepParams& p, int itchim, long long ncell, int* nfatal, hipStream_t st);
+
 // Several grids in one launch (rodas3_grids): nwork workgroups of wg lanes,
 // workgroup w on block work[w] & kGridBlockMask of grid work[w] >> kGridShift,
 // whose parameters are grids[] (device memory).  wps 1 or 4 (64 lanes only).

@@ -116,8 +116,12 @@ def jphoto_from_sza(grid, sza_col):
 
 
 def make_workload(grid, seed=0, sza="random", plume_sigma=None, plume_factor=50.0,
-                  sza_offset=0.0):
-    """Return a dict with the host arrays of one chemistry call on `grid`."""
+                  sza_offset=0.0, all_ppbm=False):
+    """Return a dict with the host arrays of one chemistry call on `grid`.
+
+    Transported species are in ppbm and the others in molec/cm^3, as the
+    Rosenbrock drivers keep them; all_ppbm=True gives every species in ppbm,
+    the units the QSSA driver converts from (chem_spack_qssa.f90:184-191)."""
     mech = load_mechanism()
     rng = np.random.Generator(np.random.Philox(seed))
     m1, m2, m3 = grid.shape
@@ -172,7 +176,7 @@ def make_workload(grid, seed=0, sza="random", plume_sigma=None, plume_factor=50.
         if s.transported:
             val = ppbv * s.weight / 28.96
         else:
-            val = ppbv * 1.0e-9 * M
+            val = ppbv * s.weight / 28.96 if all_ppbm else ppbv * 1.0e-9 * M
         sc_p.append(full(val, np.float32))
     wl["sc_p"] = sc_p
     jph = jphoto_from_sza(grid, sza_col)

@@ -153,6 +153,26 @@ int brams_chem_ros_dyndt(const brams_chem_args *args);
 int brams_chem_ros_dyndt_device(const brams_chem_dev_args *args);
 
 /*
+ * Drop-in for chem_qssa (chem_spack_qssa.f90:37-43, CHEMISTRY=1; called at
+ * chemistry.f90:399-407 with itchim = 75): every interior cell on its own,
+ * P and L of the P - L c split (rates, dratedc, fexprod, fexloss) from the
+ * state, then itchim sub-steps of length dtlt/itchim of Chang's formula with
+ * P and L held (solveur/qssa, :324-411), negatives set to 0.  Every species
+ * is read from sc_p in ppbm (non-transported ones too, :184-191) and
+ * receives sc_t += (new - old)/dtlt (:273-292; sc_t must be given for every
+ * species); sc_p is not changed.  split_method, N_DYN_CHEM, the block
+ * partition and the tolerances are not used.  J is used for FAST-TUV as for
+ * FAST-JX (the reference copies it for FAST-JX only).  Returns the number of
+ * cells where a species reached 1e20 (where solveur stops the model, :357-361;
+ * their tendencies are left unchanged), or a negative error code.  Host arrays.
+ */
+int brams_chem_qssa(const brams_chem_args *args, int32_t itchim);
+
+/* Device-resident variant of brams_chem_qssa (same semantics; the species
+ * are [47][m1*m2*m3] device arrays in sc_p and sc_t). */
+int brams_chem_qssa_device(const brams_chem_dev_args *args, int32_t itchim);
+
+/*
  * No counterpart in the reference.  brams_chem_rodas3_dyndt_device for
  * callers that cannot allocate device memory (code under stream capture,
  * tight serving loops, harnesses that forbid hipMalloc): the same step

@@ -1236,3 +1236,172 @@ int ora_scatter_cell(const brams_chem_args *a, long cell, const double *y)
     free(B);
     return 0;
 }
+
+/* ------------------------------------------------------------------ QSSA
+ * The QSSA solver of chem_spack_qssa.f90 (CHEMISTRY=1).
+ *
+ * Production and loss of the P - L c split (chem_spack_qssa.f90:247-265):
+ * rates.f90 w_r = rk_r y_a y_b; dratedc.f90 dw_rs = rk_r y_other, where the
+ * other reactant of a self-reaction is s itself (dratedc assigns that value
+ * twice) and a one-reactant rate has dw_rs = rk_r; fexprod.f90 P_s = the
+ * positive stoichiometric terms c_sr w_r, fexloss.f90 L_s = -c_sr dw_rs over
+ * the negative ones, both in reaction order.  Rows the reference never
+ * assigns (no term) are 0. */
+void ora_prod_loss(const double *rk, const double *y, double *prod, double *loss)
+{
+    double w[NR];
+    for (int r = 0; r < NR; r++) {
+        double v = rk[r];
+        if (ora_react[r][0] >= 0) v = v * y[ora_react[r][0]];
+        if (ora_react[r][1] >= 0) v = v * y[ora_react[r][1]];
+        w[r] = v;
+    }
+    for (int s = 0; s < NS; s++) {
+        double p = 0.0, l = 0.0;
+        int np = 0, nl = 0;
+        for (int t = ora_prod_start[s]; t < ora_prod_start[s + 1]; t++) {
+            const int r = ora_prod_r[t];
+            const double c = ora_prod_c[t];
+            if (c > 0.0) {
+                double term = c * w[r];
+                p = np++ ? p + term : term;
+            } else {
+                double dw = rk[r];
+                if (ora_react[r][1] >= 0) dw = dw * y[ora_react[r][0] == s ? ora_react[r][1] : ora_react[r][0]];
+                double term = (-c) * dw;
+                l = nl++ ? l + term : term;
+            }
+        }
+        prod[s] = p;
+        loss[s] = l;
+    }
+}
+
+/* One species through itchim QSSA sub-steps of length delt with P and L held
+ * for the whole call (solveur, chem_spack_qssa.f90:343-348, and qssa,
+ * :368-411, Chang et al. 1987).  cvmgm_dp / cvmgp_dp / cvmgz_dp
+ * (:421-477) are the selections written out below.  qssa:404 picks the
+ * explicit step when 100 L delt <= 1, but qssa:406 assigns the species again
+ * from cg1 and cg2 alone, so that choice never takes effect: the result is
+ * the explicit step when L = 0 and the exponential (or, for L delt >= 10, the
+ * steady-state) form otherwise. */
+double ora_qssa_species(double y, double prod, double loss, double delt, int itchim)
+{
+    for (int it = 0; it < itchim; it++) {
+        double ybid = (-loss < 0.0) ? loss : 1.0;
+        double cg1 = y + delt * (prod - loss * y);
+        double pr = prod / ybid;
+        double pr1 = pr + (y - pr) * exp(-ybid * delt);
+        double pr2 = ybid * delt - 10.0;
+        double cg2 = (pr2 >= 0.0) ? pr : pr1;
+        y = (loss == 0.0) ? cg1 : cg2;
+    }
+    return y;
+}
+
+/* gather of chem_spack_qssa.f90:161-195: the volmol of this driver (R as a
+ * REAL literal), every species converted from ppbm with the REAL product
+ * 6.02e23*1e-15 and pmar = 28.96 as a REAL literal (:100).  J is taken for
+ * FAST-TUV as for FAST-JX (:197-210 copies it for FAST-JX only and leaves it
+ * unset otherwise). */
+static void qssa_gather(const brams_chem_args *a, long o, double *y, double *temp, double *press,
+                        double *vapp, double *volmol, double *jph)
+{
+    const float cp = a->cp, cpor = a->cpor, p00 = a->p00;
+    const double pmar = (double)28.96f;
+    const double avo = (double)(6.02e23f * 1e-15f);
+    float exner = (a->pp[o] + a->pi0[o]) / cp;
+    double p = (double)exner;
+    *temp = (double)a->theta[o] * p;
+    p = pow(p, (double)cpor) * (double)p00;
+    *press = p;
+    *vapp = (double)a->rv[o];
+    *volmol = p / ((double)8.314f * *temp);
+    for (int n = 0; n < NS; n++)
+        y[n] = (double)a->sc_p[n][o] * (avo * *volmol) * pmar / (double)a->weight[n];
+    for (int q = 0; q < NP; q++) jph[q] = a->jphoto[q + (long)NP * o];
+}
+
+typedef struct {
+    const brams_chem_args *a;
+    int itchim;
+    long ncell;
+    atomic_long next;
+    atomic_long nfatal;
+} qssa_drv_t;
+
+static void *qssa_worker(void *vd)
+{
+    qssa_drv_t *D = (qssa_drv_t *)vd;
+    const brams_chem_args *a = D->a;
+    const double delt = (double)(a->dtlt / (float)D->itchim);
+    const double dtlt_i = (double)(1.0f / a->dtlt);
+    const double pmar = (double)28.96f;
+    const double avo = (double)(6.02e23f * 1e-15f);
+    double y[NS], rk[NR], jph[NP], prod[NS], loss[NS];
+    for (;;) {
+        long c0 = atomic_fetch_add(&D->next, 64);
+        if (c0 >= D->ncell) break;
+        long c1 = c0 + 64 < D->ncell ? c0 + 64 : D->ncell;
+        for (long g = c0; g < c1; g++) {
+            long o;
+            double temp, press, vapp, volmol;
+            cell_offset(a, g, &o);
+            qssa_gather(a, o, y, &temp, &press, &vapp, &volmol, jph);
+            ora_kinetic(temp, press, vapp, jph, rk);
+            ora_prod_loss(rk, y, prod, loss);
+            int fatal = 0;
+            for (int n = 0; n < NS; n++) {
+                double v = ora_qssa_species(y[n], prod[n], loss[n], delt, D->itchim);
+                if (v < 0.0) v = 0.0;               /* solveur:352-354 */
+                if (v >= 1.0e20) fatal = 1;         /* solveur:357-361 stops the model */
+                y[n] = v;
+            }
+            if (fatal) {
+                atomic_fetch_add(&D->nfatal, 1);
+                continue;
+            }
+            /* chem_spack_qssa.f90:273-292: tendency of every species */
+            for (int n = 0; n < NS; n++) {
+                double fxc = 1.0 / (avo * volmol * (pmar / (double)a->weight[n]));
+                a->sc_t[n][o] = (float)((double)a->sc_t[n][o] + (y[n] * fxc - (double)a->sc_p[n][o]) * dtlt_i);
+            }
+        }
+    }
+    return NULL;
+}
+
+/* The whole QSSA step (chem_qssa, chem_spack_qssa.f90:37-321) with itchim
+ * sub-steps (chemistry.f90:393 passes 75): sc_t of every species receives
+ * (new - old) / dtlt; sc_p is not changed.  Cells are independent.  Returns
+ * the number of cells where a species reached 1e20 (solveur stops the model
+ * there); their tendencies are left unchanged. */
+long ora_qssa(const brams_chem_args *a, int itchim, int nthreads)
+{
+    if (!a || itchim <= 0 || a->nspecies != NS || !a->sc_t) return -1;
+    for (int n = 0; n < NS; n++)
+        if (!a->sc_t[n]) return -1;
+    qssa_drv_t D;
+    D.a = a;
+    D.itchim = itchim;
+    D.ncell = (long)(a->m1 - 2) * (a->iz - a->ia + 1) * (a->jz - a->ja + 1);
+    atomic_init(&D.next, 0);
+    atomic_init(&D.nfatal, 0);
+    if (nthreads <= 0) nthreads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, qssa_worker, &D);
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    return atomic_load(&D.nfatal);
+}
+
+/* One cell's QSSA gather alone (test hook). */
+int ora_qssa_gather_cell(const brams_chem_args *a, long cell, double *y, double *temp, double *press,
+                         double *volmol)
+{
+    long o;
+    double vapp, jph[NP];
+    cell_offset(a, cell, &o);
+    qssa_gather(a, o, y, temp, press, &vapp, volmol, jph);
+    return 0;
+}

@@ -77,6 +77,20 @@ int ora_ros2_linear(int n, const double *A, const double *y0, double h, int nste
 /* ROS2's controller: dt * min(6, max(0.2, 0.9 err^-1/2)) in [dt_min, dt_max]. */
 double ora_ros2_dt_new(double dt, double max_err, double dt_min, double dt_max);
 
+/* The QSSA solver of chem_spack_qssa.f90 (CHEMISTRY=1): the whole step with
+ * itchim sub-steps (sc_t of every species += (new - old)/dtlt; sc_p is
+ * unchanged).  Returns the number of cells where a species reached 1e20
+ * (their tendencies are left unchanged), -1 on invalid arguments. */
+long ora_qssa(const brams_chem_args *a, int itchim, int nthreads);
+/* Production and loss of the P - L c split (rates, dratedc, fexprod,
+ * fexloss) of one cell. */
+void ora_prod_loss(const double *rk, const double *y, double *prod, double *loss);
+/* One species through itchim QSSA sub-steps of length delt, P and L fixed. */
+double ora_qssa_species(double y, double prod, double loss, double delt, int itchim);
+/* One cell's QSSA gather (ppbm -> molec/cm^3 for every species). */
+int ora_qssa_gather_cell(const brams_chem_args *a, long cell, double *y, double *temp, double *press,
+                         double *volmol);
+
 /* Per-block counts of the last step: attempts, accepted sub-steps. */
 int ora_block_counts(int64_t *attempts, int64_t *substeps, int max_nob);
 /* Per-block largest error accepted at dt_min (0: none) of the last step. */

@@ -7,12 +7,18 @@
 !   kinetic     src/brams/ccatt/RELACS_TUV/chem_spack_kinetic.f90:7
 !   fexchem     src/brams/ccatt/RELACS_TUV/chem_spack_fexchem.f90:10
 !   jacdchemdc  src/brams/ccatt/RELACS_TUV/chem_spack_jacdchemdc.f90:10
+!   rates, dratedc, fexprod, fexloss (the P - L c split of QSSA)
+!               src/brams/ccatt/RELACS_TUV/chem_spack_{rates,dratedc,fexprod,fexloss}.f90:9
 !   AllocIndex  src/brams/ccatt/chem_spack_utils.f90:39
 module brams_ref_harness
   use iso_c_binding
   use mod_chem_spack_kinetic,    only: kinetic
   use mod_chem_spack_fexchem,    only: fexchem
   use mod_chem_spack_jacdchemdc, only: jacdchemdc
+  use mod_chem_spack_rates,      only: rates
+  use mod_chem_spack_dratedc,    only: dratedc
+  use mod_chem_spack_fexprod,    only: fexprod
+  use mod_chem_spack_fexloss,    only: fexloss
   use Spack_utils,               only: AllocIndex, index_g, nob, maxblock_size
   implicit none
 contains
@@ -43,6 +49,24 @@ contains
     real(c_double), intent(out) :: jac(n, ns, ns)
     jac = 0.d0       ! unassigned entries stay at their allocation value 0
     call jacdchemdc(y, rk, jac, ns, 1, n, n, nr)
+  end subroutine
+
+  ! production and loss terms of chem_spack_qssa.f90:251-263 (rates, dratedc,
+  ! fexprod, fexloss in that order)
+  subroutine ref_prod_loss(n, ns, nr, y, rk, prod, loss) bind(C, name="ref_prod_loss")
+    integer(c_int), value :: n, ns, nr
+    real(c_double), intent(in)  :: y(n, ns), rk(n, nr)
+    real(c_double), intent(out) :: prod(n, ns), loss(n, ns)
+    real(c_double), allocatable :: w(:,:), dw(:,:,:)
+    allocate(w(n, nr), dw(n, nr, ns))
+    w = 0.d0; dw = 0.d0
+    prod = 0.d0      ! rows fexprod / fexloss never write stay at 0
+    loss = 0.d0
+    call rates(rk, y, w, ns, 1, n, n, nr)
+    call dratedc(rk, y, dw, ns, 1, n, n, nr)
+    call fexprod(w, prod, ns, 1, n, n, nr)
+    call fexloss(dw, loss, ns, 1, n, n, nr)
+    deallocate(w, dw)
   end subroutine
 
   ! Block partition of one grid.  Outputs are sized by the caller:

@@ -76,6 +76,12 @@ def oracle():
                                        C.POINTER(C.c_uint8)]
         lib.ora_block_counts.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int]
         lib.ora_use_external_sparse.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ora_qssa.argtypes = [C.POINTER(ChemArgs), C.c_int, C.c_int]
+        lib.ora_qssa.restype = C.c_long
+        lib.ora_prod_loss.argtypes = [_dp, _dp, _dp, _dp]
+        lib.ora_qssa_species.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
+        lib.ora_qssa_species.restype = C.c_double
+        lib.ora_qssa_gather_cell.argtypes = [C.POINTER(ChemArgs), C.c_long, _dp, _dp, _dp, _dp]
         _oracle = lib
     return _oracle
 
@@ -91,6 +97,7 @@ def ref():
         lib.ref_kinetic.argtypes = [C.c_int] * 3 + [_dp] * 5
         lib.ref_fexchem.argtypes = [C.c_int] * 3 + [_dp] * 3
         lib.ref_jacdchemdc.argtypes = [C.c_int] * 3 + [_dp] * 3
+        lib.ref_prod_loss.argtypes = [C.c_int] * 3 + [_dp] * 4
         lib.ref_allocindex.argtypes = ([C.c_int] * 8 + [C.c_float] + [C.c_int] * 3 +
                                        [_ip, _ip, _ip, _ip, _ip, _ip, _dp])
         lib.ref_sp_create.argtypes = [C.c_int, C.c_int, _ip, _ip, C.POINTER(C.c_long)]
@@ -170,6 +177,46 @@ def ref_jacobian(rk, y, chunk=16):
         lib.ref_jacdchemdc(n, NS, NR, ptr(yf), ptr(rf), ptr(out))
         outs.append(np.ascontiguousarray(out))
     return np.concatenate(outs)
+
+
+def ref_prod_loss(rk, y, chunk=16):
+    """The reference's rates, dratedc, fexprod and fexloss (the P - L c split
+    of chem_spack_qssa.f90:247-265) on cells (n, 47) -> (prod, loss)."""
+    lib = ref()
+    P, L = [], []
+    for s in range(0, len(y), chunk):
+        yf = np.asfortranarray(y[s:s + chunk]); rf = np.asfortranarray(rk[s:s + chunk])
+        n = len(yf)
+        p = np.zeros((n, NS), order="F"); l = np.zeros((n, NS), order="F")
+        lib.ref_prod_loss(n, NS, NR, ptr(yf), ptr(rf), ptr(p), ptr(l))
+        P.append(np.ascontiguousarray(p)); L.append(np.ascontiguousarray(l))
+    return np.concatenate(P), np.concatenate(L)
+
+
+def oracle_prod_loss(rk, y):
+    lib = oracle()
+    P = np.zeros_like(y); L = np.zeros_like(y)
+    for c in range(len(y)):
+        r = np.ascontiguousarray(rk[c]); yc = np.ascontiguousarray(y[c])
+        p = np.zeros(NS); l = np.zeros(NS)
+        lib.ora_prod_loss(ptr(r), ptr(yc), ptr(p), ptr(l))
+        P[c], L[c] = p, l
+    return P, L
+
+
+def qssa_species(y, prod, loss, delt, itchim):
+    return oracle().ora_qssa_species(y, prod, loss, delt, itchim)
+
+
+def run_oracle_qssa(wl, itchim=75, nthreads=0, sc_t=None):
+    """The oracle's QSSA step on a workload: returns (HostStep, rc) with
+    HostStep.sc_t the tendencies (every species), starting from sc_t (zeros
+    by default); sc_p is not changed."""
+    if sc_t is None:
+        sc_t = [np.zeros_like(x) for x in wl["sc_p"]]
+    hs = HostStep(wl, np.zeros(NS), np.zeros(NS), 2, 1, sc_t, None)
+    rc = oracle().ora_qssa(C.byref(hs.args), itchim, nthreads)
+    return hs, rc
 
 
 def ref_alloc_index(block_size, m1, m2, m3, ia, iz, ja, jz, dtlong, ndyn):

@@ -113,3 +113,22 @@ def test_noalloc_entry_refuses_grids_beyond_its_limit():
     assert a.nob == 56250
     assert L.brams_chem_rodas3_dyndt_device_noalloc(C.byref(a)) == -7
     assert b"no-allocation limit" in L.brams_chem_last_error()
+
+
+def test_qssa_arguments_are_checked_before_device_work():
+    """brams_chem_qssa(_device) refuses itchim < 1 and a missing sc_t with an
+    error, before any device work."""
+    L = abi.lib()
+    a = abi.ChemDevArgs()
+    a.m1, a.m2, a.m3, a.ia, a.iz, a.ja, a.jz = 12, 18, 18, 2, 17, 2, 17
+    a.dtlt = 60.0
+    assert L.brams_chem_qssa_device(C.byref(a), 0) == -2
+    assert b"itchim" in L.brams_chem_last_error()
+    assert L.brams_chem_qssa_device(C.byref(a), 75) == -2
+    assert b"sc_t" in L.brams_chem_last_error()
+    h = abi.ChemArgs()
+    h.m1, h.m2, h.m3, h.ia, h.iz, h.ja, h.jz = 12, 18, 18, 2, 17, 2, 17
+    h.dtlt = 60.0
+    h.nspecies = 30
+    assert L.brams_chem_qssa(C.byref(h), 75) == -2
+    assert b"nspecies" in L.brams_chem_last_error()
