@@ -18,7 +18,8 @@ Inputs stay resident in HBM; each leg A step also copies the state to a
 snapshot buffer for the accuracy check (a 322 MB device copy, ~0.1 ms).
 --path fp64 times leg B alone as the headline (no surrogate model needed).
 --path ros2 times the ROS2 solver (CHEMISTRY=3) the same way, from its own
-spin-up; --path nested the three nested grids of one model step.
+spin-up; --path qssa the QSSA solver (CHEMISTRY=1); --path nested the three
+nested grids of one model step.
 
 roofline: the kernel with the largest share of leg A's time, against its
 peak (FP64 for the RODAS3 fallback kernel, BF16 MFMA for the fused MLP);
@@ -76,6 +77,24 @@ def ros2_flop_model():
     f_att = ns + f["elimination"] + 2 * f["solve"] + f["fun"] + vec
     f_call, f_sub, _ = flop_model()
     return f_call, f_sub, f_att
+
+
+def qssa_flop_model(itchim):
+    """Algorithmic FP64 FLOP of one QSSA call per cell (chem_spack_qssa.f90):
+    the rate coefficients and conversions (as RODAS3's per-call term), the
+    P/L split, per species the setup (ybid, P/ybid, the exponent) and the
+    tendency (10), and per species and sub-step one subtraction and one FMA."""
+    with open(os.path.join(ROOT, "brams_amd", "csrc", "generated", "relacs_tuv_flops.json")) as fh:
+        f = json.load(fh)
+    ns = 47
+    f_call, _, _ = flop_model()
+    return f_call + f["prod_loss"] + 10 * ns + 3 * ns * itchim
+
+
+QSSA_BYTES_PER_CELL = 3 * 47 * 4 + 4 * 4 + 17 * 8     # sc_p in, sc_t in and out, met, J
+# PMC-measured HBM bytes per cell of qssa_cells (read + written),
+# profiles/qssa_kernel_stats_and_traffic.md
+QSSA_TRAFFIC_PER_CELL = 881 + 289
 
 
 MLP_FLOP_PER_CELL = 3 * 2 * (68 * 256 + 3 * 256 * 256 + 256 * 47)
@@ -351,6 +370,83 @@ def bench_ros2(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_qssa(args):
+    """The QSSA solver (CHEMISTRY=1, brams_chem_qssa_device, itchim 75) on the
+    headline grid with every species in ppbm: --steps timed calls with J
+    advancing as in the other paths, bracketed by synchronisations; the
+    tendencies accumulate in sc_t as the reference's do between dynamics
+    steps.  value: cell-steps/s."""
+    import numpy as np
+    import torch
+    from brams_amd import abi
+    from brams_amd.chem import DeviceStep
+    from brams_amd.workload import Grid, jphoto_from_sza, make_workload
+
+    def log(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
+
+    itchim = 75
+    grid = Grid(args.nx, args.ny, args.nz, dtlt=120.0)
+    wl = make_workload(grid, seed=0, sza="split", plume_sigma=20.0, plume_factor=50.0, all_ppbm=True)
+    K = args.steps
+    ds = DeviceStep(wl, np.zeros(47), np.zeros(47), device=0)
+    jph = [torch.from_numpy(np.ascontiguousarray(jphoto_from_sza(grid, wl["sza_col"] + 0.5 * t)
+                                                 .ravel(order="F"))).to(ds.dev) for t in range(K)]
+    log("qssa: workload %dx%dx%d = %d cells" % (args.nx, args.ny, args.nz, grid.ncell))
+    calibration_copy(args.copy_calib_mb, ds.dev, log)
+    for i in range(args.warmup):
+        ds.args.jphoto = jph[0].data_ptr()
+        ds.qssa(itchim)
+    ds.sc_t.zero_()
+    torch.cuda.synchronize()
+    wall, kms = 0.0, 0.0
+    for t in range(K):
+        ds.args.jphoto = jph[t].data_ptr()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rc = ds.qssa(itchim)
+        torch.cuda.synchronize()
+        wall += time.perf_counter() - t0
+        st = abi.stats()
+        kms += st.kernel_ms
+        log("qssa step %d: %.3f ms (%d cells at 1e20)" % (t, st.kernel_ms, rc))
+    flop = qssa_flop_model(itchim)
+    achieved = flop * grid.ncell * K / (kms * 1e-3) / 1e12
+    line = {
+        "metric": "chem cell-steps/s, QSSA (CHEMISTRY=1, itchim 75)",
+        "value": grid.ncell * K / wall, "unit": "cell-steps/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / K, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Philox generator, brams_amd/workload.py), every species in ppbm",
+        "config": {"workload": "RELACS_TUV chemistry, %dx%dx%d cells, %d QSSA calls of 120 s with 75 "
+                               "sub-steps, SZA split day/night +0.5 deg/call, NOx/VOC plume sigma 20 x50, "
+                               "seed 0" % (args.nx, args.ny, args.nz, K),
+                   "path": "QSSA (GPU)", "cells": grid.ncell, "itchim": itchim,
+                   "flop_per_cell": flop, "algorithmic_bytes_per_cell": QSSA_BYTES_PER_CELL},
+        "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": QSSA_TRAFFIC_PER_CELL * grid.ncell,
+                     "kernel": "qssa_cells",
+                     "avg_ms": kms / K, "algorithmic_bytes": QSSA_BYTES_PER_CELL * grid.ncell,
+                     "hbm_frac": QSSA_BYTES_PER_CELL * grid.ncell / (kms / K * 1e-3) / 8e12},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        cores = len(os.sched_getaffinity(0))
+        wl1 = dict(wl)
+        wl1["jphoto"] = jphoto_from_sza(grid, wl["sza_col"])
+        log("cpu baseline: the oracle's QSSA on the whole grid, %d threads" % cores)
+        t0 = time.perf_counter()
+        hs, orc = oracle_lib.run_oracle_qssa(wl1, itchim, nthreads=cores)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": grid.ncell / dt, "unit": "cell-steps/s", "cores": cores,
+                                "kind": "port",
+                                "sample": "oracle (C restatement of chem_qssa, gcc -O2 -ffp-contract=off, "
+                                          "pthreads over cells) on the whole %d-cell grid, one call "
+                                          "(%.1f s wall)" % (grid.ncell, dt)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -361,10 +457,11 @@ def main():
     ap.add_argument("--ny", type=int, default=200)
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--tol", default="baseline", choices=["baseline", "reference"])
-    ap.add_argument("--path", default="surrogate", choices=["surrogate", "fp64", "nested", "ros2"],
+    ap.add_argument("--path", default="surrogate", choices=["surrogate", "fp64", "nested", "ros2", "qssa"],
                     help="nested: the plain FP64 step on three nested grids (200x200x40 at dt 120 s, "
                          "98x98x40 at 60 s, 50x50x40 at 30 s), one call per grid and in one pass; "
-                         "ros2: the ROS2 solver (CHEMISTRY=3) rollout on the 200x200x40 workload")
+                         "ros2: the ROS2 solver (CHEMISTRY=3) rollout on the 200x200x40 workload; "
+                         "qssa: the QSSA solver (CHEMISTRY=1) on it")
     ap.add_argument("--model", default=os.path.join(ROOT, "brams_amd", "models", "relacs_tuv_surrogate.npz"))
     ap.add_argument("--spread-tol", type=float, default=None, help="override the model's spread check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -381,6 +478,8 @@ def main():
         return bench_nested(args)
     if args.path == "ros2":
         return bench_ros2(args)
+    if args.path == "qssa":
+        return bench_qssa(args)
 
     import numpy as np
     import torch

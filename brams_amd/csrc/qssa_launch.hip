@@ -20,8 +20,15 @@
 
 namespace brams {
 
+// One wave per SIMD: the rate constants and the species' state, production
+// and loss (about 270 doubles) need the whole register file, and a bound of
+// two waves costs 300 spilled registers for no gain.  Species go through the
+// sub-steps eight at a time, so each FMA chain has seven independent ones
+// beside it.  1.6M cells (ms): groups of 1 / 2 / 4 / 8 / 12 / 16 / 48 species
+// 3.13 / 2.60 / 2.27 / 2.12 / 2.13 / 2.17 / 2.61; 64, 128 or 256 lanes per
+// workgroup alike.
 constexpr int kQssaWG = 128;
-constexpr int kQssaGroup = 4;                                   // species integrated together
+constexpr int kQssaGroup = 8;                                   // species integrated together
 constexpr int kQssaPad = (MECH_NS + kQssaGroup - 1) / kQssaGroup * kQssaGroup;
 
 __global__ __launch_bounds__(kQssaWG) void qssa_cells(StepParams p, int itchim, long long ncell, int* nfatal) {
@@ -63,30 +70,40 @@ __global__ __launch_bounds__(kQssaWG) void qssa_cells(StepParams p, int itchim, 
   // (cvmgm_dp); the explicit step y + delt (P - L y) when L = 0, else the
   // steady state P/ybid when ybid delt >= 10, else P/ybid + (y - P/ybid)
   // exp(-ybid delt) (cvmgp_dp, cvmgz_dp; qssa:406 overrides the explicit
-  // choice of :404).  The branch is fixed for the call, so it is chosen once.
+  // choice of :404).  The branch is fixed for the call, and all three are one
+  // sub-step v <- (v - a) e + c with per-species coefficients:
+  //   exponential  a = c = P/ybid, e = exp(-ybid delt)
+  //   steady       a = c = P/ybid, e = 0 (v starts at P/ybid)
+  //   explicit     a = 0, e = 1, c = delt P
+  // so a wave takes one sub and one FMA per species and sub-step, whatever
+  // branch each of its cells takes.
   const double delt = (double)(p.dtlt / (float)itchim);
   bool fatal = false;
 #pragma unroll
   for (int s0 = 0; s0 < kQssaPad; s0 += kQssaGroup) {
-    double v[kQssaGroup], pr[kQssaGroup], e[kQssaGroup], dp[kQssaGroup];
-    bool expl[kQssaGroup], steady[kQssaGroup];
+    double v[kQssaGroup], a[kQssaGroup], e[kQssaGroup], c[kQssaGroup];
 #pragma unroll
     for (int q = 0; q < kQssaGroup; ++q) {
       const double l = L[s0 + q];
       const double ybid = (-l < 0.0) ? l : 1.0;
+      const double pr = P[s0 + q] / ybid;
       v[q] = y[s0 + q];
-      pr[q] = P[s0 + q] / ybid;
-      e[q] = exp(-ybid * delt);
-      dp[q] = delt * (P[s0 + q] - l * v[q]);      // only used when l == 0, where it is delt*P
-      expl[q] = (l == 0.0);
-      steady[q] = (ybid * delt - 10.0 >= 0.0);
+      if (l == 0.0) {
+        a[q] = 0.0;
+        e[q] = 1.0;
+        c[q] = delt * (P[s0 + q] - l * v[q]);
+      } else if (ybid * delt - 10.0 >= 0.0) {
+        a[q] = c[q] = pr;
+        e[q] = 0.0;
+        if (itchim > 0) v[q] = pr;
+      } else {
+        a[q] = c[q] = pr;
+        e[q] = exp(-ybid * delt);
+      }
     }
     for (int it = 0; it < itchim; ++it) {
 #pragma unroll
-      for (int q = 0; q < kQssaGroup; ++q) {
-        const double ex = pr[q] + (v[q] - pr[q]) * e[q];
-        v[q] = expl[q] ? v[q] + dp[q] : (steady[q] ? pr[q] : ex);
-      }
+      for (int q = 0; q < kQssaGroup; ++q) v[q] = fma(v[q] - a[q], e[q], c[q]);
     }
 #pragma unroll
     for (int q = 0; q < kQssaGroup; ++q) {

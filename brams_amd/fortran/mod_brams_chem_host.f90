@@ -1,8 +1,9 @@
 ! Host-array call of the chemistry library, shared by the drop-in drivers
-! mod_chem_spack_rodas3_dyndt (CHEMISTRY=4) and mod_chem_spack_ros_dyndt
-! (CHEMISTRY=3): fills brams_chem_args (include/brams_chem.h) with C_LOC
-! pointers to the caller's arrays, calls the solver's C-ABI entry
-! (brams_chem_rodas3_dyndt or brams_chem_ros_dyndt) and handles its status.
+! mod_chem_spack_rodas3_dyndt (CHEMISTRY=4), mod_chem_spack_ros_dyndt
+! (CHEMISTRY=3) and mod_chem_spack_qssa (CHEMISTRY=1): fills brams_chem_args
+! (include/brams_chem.h) with C_LOC pointers to the caller's arrays, calls
+! the solver's C-ABI entry (brams_chem_rodas3_dyndt, brams_chem_ros_dyndt or
+! brams_chem_qssa) and handles its status.
 ! The library returns a status instead of aborting: fatal errors STOP here
 ! with the library's message; blocks that hit a zero pivot or a non-finite
 ! state (where the reference's Sparse1.3 would abort the run) are reported
@@ -18,7 +19,7 @@ MODULE mod_brams_chem_host
 
   PRIVATE
 
-  PUBLIC brams_chem_host_step
+  PUBLIC brams_chem_host_step, brams_chem_host_qssa
 
   INTEGER, PARAMETER, PUBLIC :: brams_solver_rodas3 = 0, brams_solver_ros2 = 1
 
@@ -51,6 +52,12 @@ MODULE mod_brams_chem_host
        TYPE(brams_chem_args), INTENT(IN) :: args
        INTEGER(C_INT) :: rc
      END FUNCTION brams_chem_ros_dyndt
+     FUNCTION brams_chem_qssa(args, itchim) BIND(C, name='brams_chem_qssa') RESULT(rc)
+       IMPORT :: brams_chem_args, C_INT, C_INT32_T
+       TYPE(brams_chem_args), INTENT(IN) :: args
+       INTEGER(C_INT32_T), VALUE :: itchim
+       INTEGER(C_INT) :: rc
+     END FUNCTION brams_chem_qssa
      FUNCTION brams_chem_last_error() BIND(C, name='brams_chem_last_error') RESULT(p)
        IMPORT :: C_PTR
        TYPE(C_PTR) :: p
@@ -100,20 +107,11 @@ CONTAINS
     REAL(C_FLOAT), ALLOCATABLE, TARGET :: w32(:)
     INTEGER(C_INT32_T), ALLOCATABLE, TARGET :: tidx(:), nidx(:), bend(:)
     DOUBLE PRECISION, ALLOCATABLE, TARGET :: atol64(:), rtol64(:)
-    INTEGER :: n, rc, lastb, lastc
+    INTEGER :: n, rc
     LOGICAL :: par, par_n
 
-    ! interior columns from the block index of AllocIndex (j -> i -> k order)
-    lastb = nob
-    lastc = block_end(nob)
-    a%m1 = m1; a%m2 = m2; a%m3 = m3
-    a%ia = indexi(1,1);         a%ja = indexj(1,1)
-    a%iz = indexi(lastc,lastb); a%jz = indexj(lastc,lastb)
-    IF (indexk(1,1) /= 2 .OR. indexk(lastc,lastb) /= m1-1) THEN
-       PRINT *, who, ': block index does not span k=2..m1-1'
-       STOP
-    END IF
-    a%dtlt = dtlt
+    CALL fill_common(who, a, w32, m1, m2, m3, dtlt, pp, pi0, theta, rv, nspecies, nr, nr_photo, &
+                     weight, PhotojMethod, nob, block_end, indexk, indexi, indexj, cp, cpor, p00, jphoto)
     a%n_dyn_chem = N_DYN_CHEM
     SELECT CASE (TRIM(split_method))
     CASE ('SYMMETRIC');  a%split_method = 0
@@ -123,18 +121,6 @@ CONTAINS
        PRINT *, who, ': unknown split_method ', split_method
        STOP
     END SELECT
-    SELECT CASE (TRIM(PhotojMethod))
-    CASE ('FAST-TUV'); a%photoj_method = 0
-    CASE ('FAST-JX');  a%photoj_method = 1
-    CASE ('LUT');      a%photoj_method = 2
-    CASE DEFAULT;      a%photoj_method = 0
-    END SELECT
-    a%cp = cp; a%cpor = cpor; a%p00 = p00
-    a%pp = C_LOC(pp); a%pi0 = C_LOC(pi0); a%theta = C_LOC(theta); a%rv = C_LOC(rv)
-    a%jphoto = C_LOC(jphoto)
-    a%nspecies = nspecies; a%nr = nr; a%nr_photo = nr_photo
-    ALLOCATE(w32(nspecies)); w32 = weight
-    a%weight = C_LOC(w32)
     ALLOCATE(tidx(max(1,nspecies_chem_transported)), nidx(max(1,nspecies_chem_no_transported)))
     tidx(1:nspecies_chem_transported) = transp_chem_index(1:nspecies_chem_transported)
     nidx(1:nspecies_chem_no_transported) = no_transp_chem_index(1:nspecies_chem_no_transported)
@@ -180,6 +166,98 @@ CONTAINS
        PRINT *, who, ': ', rc, ' block(s) hit a zero pivot or a non-finite state'
     END IF
   END SUBROUTINE brams_chem_host_step
+
+  !========================================================================================
+  ! The fields every solver reads: grid and interior bounds (from the block
+  ! index of AllocIndex, j -> i -> k order), time step, photolysis method,
+  ! constants, meteorology, J and the molar weights (w32 keeps the REAL
+  ! copy alive for the call).
+  SUBROUTINE fill_common(who, a, w32, m1, m2, m3, dtlt, pp, pi0, theta, rv, nspecies, nr, nr_photo, &
+                         weight, PhotojMethod, nob, block_end, indexk, indexi, indexj, cp, cpor, p00, jphoto)
+    CHARACTER(LEN=*) , INTENT(IN) :: who
+    TYPE(brams_chem_args), INTENT(INOUT) :: a
+    REAL(C_FLOAT), ALLOCATABLE, TARGET, INTENT(INOUT) :: w32(:)
+    INTEGER          , INTENT(IN) :: m1, m2, m3
+    REAL             , INTENT(IN) :: dtlt
+    REAL             , INTENT(IN), TARGET :: pp(m1,m2,m3), pi0(m1,m2,m3), theta(m1,m2,m3), rv(m1,m2,m3)
+    INTEGER          , INTENT(IN) :: nspecies, nr, nr_photo
+    REAL             , INTENT(IN) :: weight(nspecies)
+    CHARACTER(LEN=*) , INTENT(IN) :: PhotojMethod
+    INTEGER          , INTENT(IN) :: nob
+    INTEGER          , INTENT(IN) :: block_end(:), indexk(:,:), indexi(:,:), indexj(:,:)
+    REAL             , INTENT(IN) :: cp, cpor, p00
+    DOUBLE PRECISION , INTENT(IN), TARGET :: jphoto(nr_photo,m1,m2,m3)
+    INTEGER :: lastb, lastc
+
+    lastb = nob
+    lastc = block_end(nob)
+    a%m1 = m1; a%m2 = m2; a%m3 = m3
+    a%ia = indexi(1,1);         a%ja = indexj(1,1)
+    a%iz = indexi(lastc,lastb); a%jz = indexj(lastc,lastb)
+    IF (indexk(1,1) /= 2 .OR. indexk(lastc,lastb) /= m1-1) THEN
+       PRINT *, who, ': block index does not span k=2..m1-1'
+       STOP
+    END IF
+    a%dtlt = dtlt
+    SELECT CASE (TRIM(PhotojMethod))
+    CASE ('FAST-TUV'); a%photoj_method = 0
+    CASE ('FAST-JX');  a%photoj_method = 1
+    CASE ('LUT');      a%photoj_method = 2
+    CASE DEFAULT;      a%photoj_method = 0
+    END SELECT
+    a%cp = cp; a%cpor = cpor; a%p00 = p00
+    a%pp = C_LOC(pp); a%pi0 = C_LOC(pi0); a%theta = C_LOC(theta); a%rv = C_LOC(rv)
+    a%jphoto = C_LOC(jphoto)
+    a%nspecies = nspecies; a%nr = nr; a%nr_photo = nr_photo
+    ALLOCATE(w32(nspecies)); w32 = weight
+    a%weight = C_LOC(w32)
+  END SUBROUTINE fill_common
+
+  !========================================================================================
+  ! One QSSA call (chem_qssa) on the caller's host arrays: every species'
+  ! sc_p is read (ppbm) and sc_t receives the tendency; itchim sub-steps.
+  SUBROUTINE brams_chem_host_qssa(who, m1, m2, m3, itchim, dtlt, pp, pi0, theta, rv, jphoto, &
+                                  nspecies, nr, nr_photo, weight, PhotojMethod, chem1_g, nob, &
+                                  block_end, indexk, indexi, indexj, cp, cpor, p00)
+    CHARACTER(LEN=*) , INTENT(IN) :: who
+    INTEGER          , INTENT(IN) :: m1, m2, m3, itchim
+    REAL             , INTENT(IN) :: dtlt
+    REAL             , INTENT(IN), TARGET :: pp(m1,m2,m3), pi0(m1,m2,m3), theta(m1,m2,m3), rv(m1,m2,m3)
+    INTEGER          , INTENT(IN) :: nspecies, nr, nr_photo
+    DOUBLE PRECISION , INTENT(IN), TARGET :: jphoto(nr_photo,m1,m2,m3)
+    REAL             , INTENT(IN) :: weight(nspecies)
+    CHARACTER(LEN=*) , INTENT(IN) :: PhotojMethod
+    TYPE (chem1_vars), INTENT(INOUT) :: chem1_g(nspecies)
+    INTEGER          , INTENT(IN) :: nob
+    INTEGER          , INTENT(IN) :: block_end(:), indexk(:,:), indexi(:,:), indexj(:,:)
+    REAL             , INTENT(IN) :: cp, cpor, p00
+
+    TYPE(brams_chem_args) :: a
+    TYPE(C_PTR), ALLOCATABLE, TARGET :: tab_p(:), tab_t(:)
+    REAL(C_FLOAT), ALLOCATABLE, TARGET :: w32(:)
+    INTEGER :: n, rc
+
+    CALL fill_common(who, a, w32, m1, m2, m3, dtlt, pp, pi0, theta, rv, nspecies, nr, nr_photo, &
+                     weight, PhotojMethod, nob, block_end, indexk, indexi, indexj, cp, cpor, p00, jphoto)
+    a%n_dyn_chem = 1
+    a%split_method = 0
+    ALLOCATE(tab_p(nspecies), tab_t(nspecies))
+    DO n = 1, nspecies
+       tab_p(n) = C_LOC(chem1_g(n)%sc_p(1,1,1))
+       tab_t(n) = C_LOC(chem1_g(n)%sc_t(1))
+    END DO
+    a%sc_p = C_LOC(tab_p)
+    a%sc_t = C_LOC(tab_t)
+    a%sc_t_dyn = C_NULL_PTR
+    a%nob = nob
+    rc = brams_chem_qssa(a, INT(itchim, C_INT32_T))
+    IF (rc < 0) THEN
+       PRINT *, who, ': ', TRIM(last_error())
+       STOP
+    ELSE IF (rc > 0) THEN
+       PRINT *, who, ': ', rc, ' cell(s) reached 1e20, tendencies left unchanged'
+    END IF
+  END SUBROUTINE brams_chem_host_qssa
 
   FUNCTION last_error() RESULT(msg)
     CHARACTER(LEN=512) :: msg

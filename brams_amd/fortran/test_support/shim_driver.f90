@@ -1,13 +1,16 @@
 ! Test driver of the Fortran drop-ins: reads one chemistry call from a stream
 ! file written by tests/test_gpu_fortran_shim.py, calls chem_rodas3_dyndt
-! exactly as chemistry.f90:443-453 does (or, with a third argument "ros2",
-! chem_ros_dyndt as chemistry.f90:430-440 does), and writes the updated state.
+! exactly as chemistry.f90:443-453 does (with a third argument "ros2",
+! chem_ros_dyndt as chemistry.f90:430-440 does; with "qssa", chem_qssa as
+! chemistry.f90:399-407 does, itchim 75), and writes the updated state (sc_p,
+! or for qssa the tendencies sc_t) and last_accepted_dt.
 PROGRAM shim_driver
   USE mem_chem1, ONLY: chem1_vars
   USE mem_spack, ONLY: spack_type, spack_type_2d
   USE Extras, ONLY: ext3d
   USE mod_chem_spack_rodas3_dyndt, ONLY: chem_rodas3_dyndt
   USE mod_chem_spack_ros_dyndt, ONLY: chem_ros_dyndt
+  USE mod_chem_spack_qssa, ONLY: chem_qssa
   IMPLICIT NONE
   INTEGER :: m1, m2, m3, nob, mbs, nsp, nr, nrp, ntr, nnt, ndyn, n, u
   REAL :: dtlt, cp, cpor, p00
@@ -48,7 +51,15 @@ PROGRAM shim_driver
   photoj = 'FAST-TUV'
   split = 'SYMMETRIC'
   gnz = .FALSE.
-  IF (TRIM(solver) == 'ros2') THEN
+  IF (TRIM(solver) == 'qssa') THEN
+     DO n = 1, nsp
+        ALLOCATE(chem1_g(n)%sc_t(m1*m2*m3))
+        chem1_g(n)%sc_t = 0.0
+     END DO
+     CALL chem_qssa(m1,m2,m3,75,nr,0,dtlt,pp,pi0,theta,rv, &
+                    cosz,jphoto,nsp,nrp,weight,photoj,chem1_g, &
+                    nob,mbs,block_end,ik,ii,ij,kij,cp,cpor,p00,att)
+  ELSE IF (TRIM(solver) == 'ros2') THEN
      CALL chem_ros_dyndt(m1,m2,m3,dtlt,pp,pi0,theta,rv,dn0,cosz,rcp, &
                          nsp,nr,nrp,weight,photoj,200,ntr,nnt,tidx,nidx,chem1_g,nob,mbs, &
                          block_end,ik,ii,ij,kij,last_dt,spack,spack_2d,atol,rtol,gnz, &
@@ -61,7 +72,11 @@ PROGRAM shim_driver
   END IF
   OPEN(newunit=u, file=TRIM(fout), access='stream', form='unformatted', status='replace')
   DO n = 1, nsp
-     WRITE(u) chem1_g(n)%sc_p
+     IF (TRIM(solver) == 'qssa') THEN
+        WRITE(u) chem1_g(n)%sc_t
+     ELSE
+        WRITE(u) chem1_g(n)%sc_p
+     END IF
   END DO
   WRITE(u) last_dt
   CLOSE(u)

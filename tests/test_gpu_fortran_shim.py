@@ -84,3 +84,31 @@ def test_fortran_shim_matches_c_abi_and_oracle(tmp_path, solver):
     hs, st, mask, orc = oracle_lib.run_oracle(wl, atol, rtol, solver=solver)
     rep = parity_report(fort, hs.sc_p, grid, wl)
     assert rep["nfail"] == 0, rep
+
+
+def test_fortran_qssa_matches_c_abi_and_oracle(tmp_path):
+    """chem_qssa (mod_chem_spack_qssa) with the reference's argument list:
+    tendencies bit-identical to the C-ABI's, parity with the oracle."""
+    from brams_amd.chem import run_workload_qssa_host
+    from test_gpu_qssa import compare
+    assert os.path.exists(DRIVER), "Fortran driver not built"
+    grid = Grid(10, 8, 9)
+    wl = make_workload(grid, seed=11, sza="split", all_ppbm=True)
+    atol, rtol = tolerances("baseline")
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write_input(fin, wl, atol, rtol)
+    print("case fortran shim qssa 10x8x9 seed=11", flush=True)
+    r = subprocess.run([DRIVER, fin, fout, "qssa"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    n3 = grid.m1 * grid.m2 * grid.m3
+    raw = open(fout, "rb").read()
+    sc = np.frombuffer(raw[:47 * n3 * 4], np.float32).reshape(47, n3)
+    fort = [np.asfortranarray(sc[s].reshape(grid.shape, order="F")) for s in range(47)]
+    py, rc = run_workload_qssa_host(wl)
+    assert rc == 0
+    for a, b in zip(fort, py):
+        assert np.array_equal(a, b)
+    hs, orc = oracle_lib.run_oracle_qssa(wl)
+    assert orc == 0
+    _, nfail = compare(grid, wl, fort, hs)
+    assert nfail == 0
