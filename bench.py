@@ -18,8 +18,9 @@ Inputs stay resident in HBM; each leg A step also copies the state to a
 snapshot buffer for the accuracy check (a 322 MB device copy, ~0.1 ms).
 --path fp64 times leg B alone as the headline (no surrogate model needed).
 --path ros2 times the ROS2 solver (CHEMISTRY=3) the same way, from its own
-spin-up; --path qssa the QSSA solver (CHEMISTRY=1); --path nested the three
-nested grids of one model step.
+spin-up; --path qssa the QSSA solver (CHEMISTRY=1); --path tuv the FAST-TUV
+j-value mapping into jphoto; --path nested the three nested grids of one
+model step.
 
 roofline: the kernel with the largest share of leg A's time, against its
 peak (FP64 for the RODAS3 fallback kernel, BF16 MFMA for the fused MLP);
@@ -95,6 +96,9 @@ QSSA_BYTES_PER_CELL = 3 * 47 * 4 + 4 * 4 + 17 * 8     # sc_p in, sc_t in and out
 # PMC-measured HBM bytes per cell of qssa_cells (read + written),
 # profiles/qssa_kernel_stats_and_traffic.md
 QSSA_TRAFFIC_PER_CELL = 881 + 289
+# the same for tuv_map_interior, bytes per interior grid point at kj 80, kz
+# 125 with 17 reactions mapped (profiles/tuv_jphoto_mapping.md)
+TUV_TRAFFIC_PER_POINT = 254
 
 
 MLP_FLOP_PER_CELL = 3 * 2 * (68 * 256 + 3 * 256 * 256 + 256 * 47)
@@ -447,6 +451,64 @@ def bench_qssa(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_tuv(args):
+    """The FAST-TUV output mapping and RELACS J relations
+    (brams_chem_tuv_jphoto_device) on the headline grid with TUV's own
+    sizes (kj = 80 reactions, kz = 125 levels, 17 of them mapped): --steps
+    timed calls on resident device arrays.  value: grid points per second;
+    the roofline is HBM against the bytes the mapping must move."""
+    import numpy as np
+    import torch
+    from brams_amd import abi
+    from brams_amd.chem import tuv_jphoto_device
+    from brams_amd.workload import Grid
+
+    def log(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
+
+    grid = Grid(args.nx, args.ny, args.nz)
+    kj, kz = 80, 125
+    ni, nj = grid.iz - grid.ia + 1, grid.jz - grid.ja + 1
+    xref = np.zeros(kj, np.int32)
+    xref[np.arange(2, 2 + 4 * 17, 4)] = np.arange(1, 18)     # 17 TUV reactions, one per rate
+    g = torch.Generator(device="cuda").manual_seed(0)
+    valj = torch.rand(ni * nj * kj * kz, dtype=torch.float64, device="cuda", generator=g) * 1e-2
+    n3d = grid.m1 * grid.m2 * grid.m3
+    jph = torch.rand(17 * n3d, dtype=torch.float64, device="cuda", generator=g) * 1e-3
+    calibration_copy(args.copy_calib_mb, torch.device("cuda", 0), log)
+    for i in range(args.warmup):
+        tuv_jphoto_device(grid, valj, kj, kz, xref, jph)
+    torch.cuda.synchronize()
+    wall, kms = 0.0, 0.0
+    for t in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tuv_jphoto_device(grid, valj, kj, kz, xref, jph)
+        torch.cuda.synchronize()
+        wall += time.perf_counter() - t0
+        kms += abi.stats().kernel_ms
+    K = args.steps
+    npts = grid.m1 * ni * nj
+    # mapped slices read (levels 2..m1 from valj), jphoto written at every
+    # interior point, J14/J16 read-modified-written elsewhere (3 read + 2 written)
+    nbound = n3d - npts
+    algo = 17 * ni * nj * (grid.m1 - 1) * 8 + 17 * npts * 8 + nbound * 5 * 8
+    achieved = algo / (kms / K * 1e-3) / 1e9
+    line = {
+        "metric": "photolysis mapping grid points/s (FAST-TUV output -> jphoto + RELACS J relations)",
+        "value": npts * K / wall, "unit": "points/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (torch.rand j-values)",
+        "config": {"workload": "TUV j-values of a %dx%dx%d grid, kj 80, kz 125, 17 reactions mapped"
+                               % (args.nx, args.ny, args.nz), "points": npts},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": TUV_TRAFFIC_PER_POINT * npts,
+                     "kernel": "tuv_map_interior",
+                     "avg_ms": kms / K, "algorithmic_bytes": algo},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -457,11 +519,12 @@ def main():
     ap.add_argument("--ny", type=int, default=200)
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--tol", default="baseline", choices=["baseline", "reference"])
-    ap.add_argument("--path", default="surrogate", choices=["surrogate", "fp64", "nested", "ros2", "qssa"],
+    ap.add_argument("--path", default="surrogate", choices=["surrogate", "fp64", "nested", "ros2", "qssa", "tuv"],
                     help="nested: the plain FP64 step on three nested grids (200x200x40 at dt 120 s, "
                          "98x98x40 at 60 s, 50x50x40 at 30 s), one call per grid and in one pass; "
                          "ros2: the ROS2 solver (CHEMISTRY=3) rollout on the 200x200x40 workload; "
-                         "qssa: the QSSA solver (CHEMISTRY=1) on it")
+                         "qssa: the QSSA solver (CHEMISTRY=1) on it; tuv: the FAST-TUV j-value "
+                         "mapping into jphoto")
     ap.add_argument("--model", default=os.path.join(ROOT, "brams_amd", "models", "relacs_tuv_surrogate.npz"))
     ap.add_argument("--spread-tol", type=float, default=None, help="override the model's spread check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -480,6 +543,8 @@ def main():
         return bench_ros2(args)
     if args.path == "qssa":
         return bench_qssa(args)
+    if args.path == "tuv":
+        return bench_tuv(args)
 
     import numpy as np
     import torch

@@ -21,7 +21,7 @@ _ip = C.POINTER(C.c_int32)
 EXPORTS = ["brams_chem_init", "brams_chem_rodas3_dyndt", "brams_chem_rodas3_dyndt_device",
            "brams_chem_rodas3_dyndt_device_noalloc", "brams_chem_rodas3_dyndt_device_grids",
            "brams_chem_accum_device", "brams_chem_ros_dyndt", "brams_chem_ros_dyndt_device",
-           "brams_chem_qssa", "brams_chem_qssa_device",
+           "brams_chem_qssa", "brams_chem_qssa_device", "brams_chem_tuv_jphoto_device",
            "brams_chem_alloc_index", "brams_chem_get_stats", "brams_chem_last_error",
            "brams_chem_finalize", "brams_chem_block_status", "brams_chem_block_thresholds",
            "brams_chem_set_form", "brams_chem_get_tuning"]
@@ -45,6 +45,17 @@ class ChemArgs(C.Structure):
         ("nob", C.c_int32), ("maxblock_size", C.c_int32),
         ("block_end", _ip), ("last_accepted_dt", _dp),
         ("atol", _dp), ("rtol", _dp),
+    ]
+
+
+class TuvMapArgs(C.Structure):
+    """brams_tuv_map_args"""
+    _fields_ = [
+        ("m1", C.c_int32), ("m2", C.c_int32), ("m3", C.c_int32),
+        ("ia", C.c_int32), ("iz", C.c_int32), ("ja", C.c_int32), ("jz", C.c_int32),
+        ("kj", C.c_int32), ("kz", C.c_int32),
+        ("valj", C.c_void_p), ("xref", C.POINTER(C.c_int32)), ("nr_photo", C.c_int32),
+        ("jphoto", C.c_void_p), ("stream", C.c_void_p),
     ]
 
 
@@ -112,6 +123,7 @@ def lib():
         L.brams_chem_ros_dyndt_device.argtypes = [C.POINTER(ChemDevArgs)]
         L.brams_chem_qssa.argtypes = [C.POINTER(ChemArgs), C.c_int32]
         L.brams_chem_qssa_device.argtypes = [C.POINTER(ChemDevArgs), C.c_int32]
+        L.brams_chem_tuv_jphoto_device.argtypes = [C.POINTER(TuvMapArgs)]
         L.brams_chem_alloc_index.argtypes = [C.c_int32] * 6 + [_ip, C.c_int32]
         L.brams_chem_get_stats.argtypes = [C.POINTER(ChemStats)]
         L.brams_chem_last_error.restype = C.c_char_p

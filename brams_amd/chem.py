@@ -144,6 +144,28 @@ def run_workload_qssa_host(wl, itchim=75, sc_t=None):
     return sct, rc
 
 
+def tuv_jphoto_device(grid, valj, kj, kz, xref, jphoto, stream=None):
+    """The FAST-TUV output mapping and RELACS's J relations
+    (ModTuvDriver2.7.f90:1302-1332, 1363-1368) on device memory: valj a
+    float64 tensor holding valJLoc(ia:iz, ja:jz, kj, kz) in Fortran order,
+    xref the TUV cross-reference (kj ints: 1-based jphoto index, 0 unused),
+    jphoto a float64 tensor holding (17, m1, m2, m3) in Fortran order,
+    updated in place."""
+    L = abi.lib()
+    x = np.ascontiguousarray(xref, dtype=np.int32)
+    if x.size != kj:
+        raise abi.ChemError("xref must hold kj entries")
+    a = abi.TuvMapArgs()
+    a.m1, a.m2, a.m3, a.ia, a.iz, a.ja, a.jz = grid.m1, grid.m2, grid.m3, grid.ia, grid.iz, grid.ja, grid.jz
+    a.kj, a.kz = kj, kz
+    a.valj = valj.data_ptr()
+    a.xref = x.ctypes.data_as(C.POINTER(C.c_int32))
+    a.nr_photo = 17
+    a.jphoto = jphoto.data_ptr()
+    a.stream = stream
+    return abi.check(L.brams_chem_tuv_jphoto_device(C.byref(a)), "brams_chem_tuv_jphoto_device")
+
+
 def run_workload_host(wl, atol, rtol, split_method="SYMMETRIC", n_dyn_chem=1, sc_t=None,
                       sc_t_dyn=None, solver="rodas3"):
     """Convenience: run chem_rodas3_dyndt (or chem_ros_dyndt) on a

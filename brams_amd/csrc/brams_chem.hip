@@ -25,6 +25,7 @@
 #include "rodas3_launch.h"
 #include "rodas3_slab_launch.h"
 #include "sched_kernels.h"
+#include "jphoto_kernels.h"
 #include "sur_step_kernels.h"
 
 namespace {
@@ -1320,6 +1321,54 @@ int brams_chem_qssa(const brams_chem_args* a, int32_t itchim) {
     HIP_TRY(hipMemcpyAsync(a->sc_t[s], tc + s * fsz, fsz, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   return nfatal;
+}
+
+int brams_chem_tuv_jphoto_device(const brams_tuv_map_args* a) {
+  if (!a) return fail(-2, "null args");
+  if (a->m1 < 3 || a->m2 < 1 || a->m3 < 1 || a->ia < 1 || a->iz > a->m2 || a->ia > a->iz || a->ja < 1 ||
+      a->jz > a->m3 || a->ja > a->jz)
+    return fail(-2, "bad grid %d %d %d / interior %d..%d %d..%d", a->m1, a->m2, a->m3, a->ia, a->iz, a->ja, a->jz);
+  if (a->nr_photo != MECH_NPHOT) return fail(-2, "nr_photo %d differs from RELACS_TUV %d", a->nr_photo, MECH_NPHOT);
+  if (a->kj < 1 || a->kz < a->m1 - 1)
+    return fail(-2, "valj must hold kj >= 1 reactions and kz >= m1-1 = %d levels (kj %d, kz %d)", a->m1 - 1,
+                a->kj, a->kz);
+  if (!a->valj || !a->xref || !a->jphoto) return fail(-2, "null pointer");
+  brams::TuvMapParams p{};
+  p.m1 = a->m1; p.m2 = a->m2; p.m3 = a->m3;
+  p.ia = a->ia; p.iz = a->iz; p.ja = a->ja; p.jz = a->jz;
+  p.kj = a->kj; p.kz = a->kz;
+  p.valj = a->valj;
+  p.jphoto = a->jphoto;
+  for (int n = 0; n < MECH_NPHOT; ++n) p.src[n] = -1;
+  for (int k = 0; k < a->kj; ++k) {            // ascending: a later reaction overwrites an earlier one
+    const int x = a->xref[k];
+    if (x < 0 || x > MECH_NPHOT) return fail(-2, "xref(%d) = %d outside 0..%d", k + 1, x, MECH_NPHOT);
+    if (x > 0) p.src[x - 1] = k;
+  }
+  int rc;
+  if (!g.init && (rc = brams_chem_init(0))) return rc;
+  hipStream_t st = (hipStream_t)a->stream;
+  const long long ni = a->iz - a->ia + 1, nj = a->jz - a->ja + 1;
+  const long long npts = (long long)a->m1 * ni * nj, n3d = (long long)a->m1 * a->m2 * a->m3;
+  HIP_TRY(hipEventRecord(g.ev0, st));
+  const unsigned tiles = (unsigned)((npts + 255) / 256), chunk = (tiles + brams::kXcds - 1) / brams::kXcds;
+  hipLaunchKernelGGL(brams::tuv_map_interior, dim3(chunk * brams::kXcds), dim3(256), 0, st, p, npts, chunk);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(brams::jphoto_relations_outside, dim3((unsigned)((n3d + 255) / 256)), dim3(256), 0, st, p,
+                     n3d);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(g.ev1, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  brams_chem_stats s{};
+  s.cells = npts;
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+  s.kernel_ms = ms;
+  s.main_ms = ms;
+  s.main_cells = npts;
+  s.kernel_form = -1;
+  g.stats = s;
+  return 0;
 }
 
 int brams_chem_block_status(int32_t* attempts, int32_t* substeps, int32_t* failed,

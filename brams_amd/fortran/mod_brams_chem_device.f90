@@ -9,6 +9,8 @@
 ! (include/brams_chem.h) through ISO_C_BINDING:
 !   chem_rodas3_dyndt_device -> brams_chem_rodas3_dyndt_device
 !   chem_accum_device        -> brams_chem_accum_device (chemistry.f90:528-604)
+!   chem_tuv_jphoto_device   -> brams_chem_tuv_jphoto_device (the FAST-TUV output
+!                               mapping into jphoto, ModTuvDriver2.7.f90:1302-1368)
 ! Device arrays are passed as C pointers: the species as one
 ! [nspecies][m1*m2*m3] REAL buffer each for sc_p, sc_t and sc_t_dyn.
 MODULE mod_brams_chem_device
@@ -19,7 +21,7 @@ MODULE mod_brams_chem_device
 
   PRIVATE
 
-  PUBLIC brams_chem_dev_args, chem_rodas3_dyndt_device, chem_accum_device
+  PUBLIC brams_chem_dev_args, chem_rodas3_dyndt_device, chem_accum_device, chem_tuv_jphoto_device
 
   INTEGER, PARAMETER, PUBLIC :: brams_split_symmetric = 0, brams_split_sequential = 1, &
                                 brams_split_parallel = 2
@@ -40,7 +42,23 @@ MODULE mod_brams_chem_device
      TYPE(C_PTR)        :: stream
   END TYPE brams_chem_dev_args
 
+  ! mirror of brams_tuv_map_args (include/brams_chem.h)
+  TYPE, BIND(C) :: brams_tuv_map_args
+     INTEGER(C_INT32_T) :: m1, m2, m3, ia, iz, ja, jz
+     INTEGER(C_INT32_T) :: kj, kz
+     TYPE(C_PTR)        :: valj
+     TYPE(C_PTR)        :: xref
+     INTEGER(C_INT32_T) :: nr_photo
+     TYPE(C_PTR)        :: jphoto
+     TYPE(C_PTR)        :: stream
+  END TYPE brams_tuv_map_args
+
   INTERFACE
+     FUNCTION brams_chem_tuv_jphoto_device(args) BIND(C, name='brams_chem_tuv_jphoto_device') RESULT(rc)
+       IMPORT :: brams_tuv_map_args, C_INT
+       TYPE(brams_tuv_map_args), INTENT(IN) :: args
+       INTEGER(C_INT) :: rc
+     END FUNCTION brams_chem_tuv_jphoto_device
      FUNCTION brams_chem_rodas3_dyndt_device(args) BIND(C, name='brams_chem_rodas3_dyndt_device') RESULT(rc)
        IMPORT :: brams_chem_dev_args, C_INT
        TYPE(brams_chem_dev_args), INTENT(IN) :: args
@@ -98,6 +116,32 @@ CONTAINS
                                  INT(nspecies_chem_transported, C_INT32_T), idx, sc_t_dyn, sc_t, stream)
     IF (rc < 0) CALL fatal('chem_accum_device', rc)
   END SUBROUTINE chem_accum_device
+
+  !========================================================================================
+  ! TUV's j-values to jphoto on the device (ModTuvDriver2.7.f90:1302-1332) and
+  ! RELACS's J14/J16 relations (:1363-1368): valjloc is the device copy of
+  ! valJLoc(ia:iz,ja:jz,kj,kz) (double), xref TUV's cross-reference (host;
+  ! 1-based jphoto index of reaction k, 0 = unused), jphoto the device
+  ! (nr_photo,m1,m2,m3) array, updated in place.
+  SUBROUTINE chem_tuv_jphoto_device(m1, m2, m3, ia, iz, ja, jz, kj, kz, valjloc, xref, nr_photo, &
+                                    jphoto, stream)
+    INTEGER, INTENT(IN) :: m1, m2, m3, ia, iz, ja, jz, kj, kz, nr_photo
+    INTEGER, INTENT(IN) :: xref(kj)
+    TYPE(C_PTR), INTENT(IN) :: valjloc, jphoto, stream
+    TYPE(brams_tuv_map_args) :: a
+    INTEGER(C_INT32_T), TARGET :: x(kj)
+    INTEGER(C_INT) :: rc
+    x = INT(xref, C_INT32_T)
+    a%m1 = m1; a%m2 = m2; a%m3 = m3; a%ia = ia; a%iz = iz; a%ja = ja; a%jz = jz
+    a%kj = kj; a%kz = kz
+    a%valj = valjloc
+    a%xref = C_LOC(x)
+    a%nr_photo = nr_photo
+    a%jphoto = jphoto
+    a%stream = stream
+    rc = brams_chem_tuv_jphoto_device(a)
+    IF (rc < 0) CALL fatal('chem_tuv_jphoto_device', rc)
+  END SUBROUTINE chem_tuv_jphoto_device
 
   !========================================================================================
   SUBROUTINE fatal(who, rc)

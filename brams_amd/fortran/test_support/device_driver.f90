@@ -1,8 +1,10 @@
 ! Test driver of the device-resident coupling (mod_brams_chem_device): reads a
 ! grid, its state and the dynamic tendencies of every dynamics step from a
 ! stream file written by tests/test_gpu_fortran_device.py, keeps everything in
-! device memory, and runs the PARALLEL-split sequence of chemistry.f90:383-460
-! for nchem chemistry steps of N_DYN_CHEM dynamics steps each:
+! device memory, maps TUV j-values into jphoto when the file holds them
+! (kj > 0, chem_tuv_jphoto_device), and runs the PARALLEL-split sequence of
+! chemistry.f90:383-460 for nchem chemistry steps of N_DYN_CHEM dynamics
+! steps each:
 !   every dynamics step:      sc_t <- that step's tendency; chem_accum ntask 1
 !   every N_DYN_CHEM-th step: chem_rodas3_dyndt; chem_accum ntask 2
 ! then writes sc_p and last_accepted_dt.  Device memory comes from the HIP
@@ -36,20 +38,21 @@ PROGRAM device_driver
      END FUNCTION hipDeviceSynchronize
   END INTERFACE
   INTEGER(C_INT), PARAMETER :: h2d = 1, d2h = 2
-  INTEGER :: m1, m2, m3, ia, iz, ja, jz, nob, nsp, ntr, ndyn, nchem, c, d, u, nf, n3
+  INTEGER :: m1, m2, m3, ia, iz, ja, jz, nob, nsp, ntr, ndyn, nchem, c, d, u, nf, n3, kj, kz
   REAL :: dtlt, cp, cpor, p00
   REAL(C_FLOAT), ALLOCATABLE, TARGET :: pp(:), pi0(:), theta(:), rv(:), weight(:), sc_p(:), tend(:), zero(:)
-  REAL(C_DOUBLE), ALLOCATABLE, TARGET :: jphoto(:), last_dt(:), atol(:), rtol(:)
+  REAL(C_DOUBLE), ALLOCATABLE, TARGET :: jphoto(:), last_dt(:), atol(:), rtol(:), valj(:)
   INTEGER(C_INT32_T), ALLOCATABLE, TARGET :: transported(:)
-  INTEGER, ALLOCATABLE :: tidx(:)
+  INTEGER, ALLOCATABLE :: tidx(:), xref(:)
   TYPE(C_PTR) :: d_pp, d_pi0, d_theta, d_rv, d_jph, d_w, d_tr, d_scp, d_sct, d_sctd, d_dt, d_atol, d_rtol
+  TYPE(C_PTR) :: d_valj
   TYPE(brams_chem_dev_args) :: a
   CHARACTER(LEN=256) :: fin, fout
 
   CALL get_command_argument(1, fin)
   CALL get_command_argument(2, fout)
   OPEN(newunit=u, file=TRIM(fin), access='stream', form='unformatted', status='old')
-  READ(u) m1, m2, m3, ia, iz, ja, jz, nob, nsp, ntr, ndyn, nchem
+  READ(u) m1, m2, m3, ia, iz, ja, jz, nob, nsp, ntr, ndyn, nchem, kj, kz
   READ(u) dtlt, cp, cpor, p00
   n3 = m1 * m2 * m3
   ALLOCATE(pp(n3), pi0(n3), theta(n3), rv(n3), weight(nsp), transported(nsp), jphoto(17 * n3))
@@ -80,6 +83,14 @@ PROGRAM device_driver
   a%nob = nob; a%last_accepted_dt = d_dt; a%atol = d_atol; a%rtol = d_rtol
   a%stream = C_NULL_PTR
 
+  IF (kj > 0) THEN                 ! TUV j-values -> jphoto on the device
+     ALLOCATE(valj((iz - ia + 1) * (jz - ja + 1) * kj * kz), xref(kj))
+     READ(u) valj, xref
+     d_valj = upload(C_LOC(valj), 8_C_SIZE_T * SIZE(valj))
+     CALL chem_tuv_jphoto_device(m1, m2, m3, ia, iz, ja, jz, kj, kz, d_valj, xref, 17, d_jph, C_NULL_PTR)
+     CALL check(hipFree(d_valj), 'hipFree')
+  END IF
+
   DO c = 1, nchem
      DO d = 1, ndyn
         READ(u) tend
@@ -96,7 +107,8 @@ PROGRAM device_driver
   CALL check(hipMemcpy(C_LOC(last_dt), d_dt, 8_C_SIZE_T * nob, d2h), 'hipMemcpy last_accepted_dt')
   CALL check(hipMemcpy(C_LOC(tend), d_sctd, 4_C_SIZE_T * nsp * n3, d2h), 'hipMemcpy sc_t_dyn')
   OPEN(newunit=u, file=TRIM(fout), access='stream', form='unformatted', status='replace')
-  WRITE(u) sc_p, last_dt, tend
+  CALL check(hipMemcpy(C_LOC(jphoto), d_jph, 8_C_SIZE_T * 17 * n3, d2h), 'hipMemcpy jphoto')
+  WRITE(u) sc_p, last_dt, tend, jphoto
   CLOSE(u)
   CALL check(hipFree(d_pp), 'hipFree')
   CALL check(hipFree(d_pi0), 'hipFree')

@@ -190,6 +190,32 @@ int brams_chem_qssa_device(const brams_chem_dev_args *args, int32_t itchim);
 int brams_chem_rodas3_dyndt_device_noalloc(const brams_chem_dev_args *args);
 
 /*
+ * The FAST-TUV output mapping of tuvDriver (ModTuvDriver2.7.f90:1302-1332) and
+ * RELACS's J relations (:1363-1368) on device arrays, the photolysis input of
+ * the chemistry step (called every radfrq, chemistry.f90:338-375).  For every
+ * TUV reaction k (ascending) with xref[k] /= 0, over the interior columns:
+ * jphoto(xref[k], m+1, i, j) = valJLoc(i, j, k, m) for m = 1..m1-1 and
+ * jphoto(xref[k], 1, i, j) = jphoto(xref[k], 2, i, j); then at every point
+ * J14 = 0.962055 J13 + 0.0106247 J9 and J16 = 12 J13 + 208 J15 (REAL-literal
+ * constants, each product and sum rounded).  Entries no reaction maps keep
+ * their values.  valj is valJLoc (:593, :672, :1263), the double copy of
+ * TUV's j-values; xref is TUV's reaction cross-reference (read from
+ * <mechanism>xreference.dat by ModTuv2.7.f90:17419-17426).  Stream-ordered;
+ * returns when the kernels have run (their time is in brams_chem_get_stats).
+ */
+typedef struct brams_tuv_map_args {
+    int32_t m1, m2, m3, ia, iz, ja, jz;
+    int32_t kj, kz;             /* TUV reactions and levels of valj (ModTuv2.7.f90:79-86: 80, 125) */
+    const double *valj;         /* valJLoc(ia:iz, ja:jz, kj, kz), Fortran order, device */
+    const int32_t *xref;        /* [kj] host: jphoto index (1-based) of TUV reaction k, 0 = unused */
+    int32_t nr_photo;           /* photolysis rates of the mechanism (17) */
+    double *jphoto;             /* (nr_photo, m1, m2, m3), device, updated in place */
+    void *stream;               /* hipStream_t, NULL for the default stream */
+} brams_tuv_map_args;
+
+int brams_chem_tuv_jphoto_device(const brams_tuv_map_args *args);
+
+/*
  * chem_accum (chemistry.f90:528-604) on device arrays, for a model that keeps
  * the tracers in HBM.  With the PARALLEL split and N_DYN_CHEM > 1 the driver
  * accumulates the mean dynamic tendency of the transported species over a

@@ -1405,3 +1405,40 @@ int ora_qssa_gather_cell(const brams_chem_args *a, long cell, double *y, double 
     qssa_gather(a, o, y, temp, press, &vapp, volmol, jph);
     return 0;
 }
+
+/* ------------------------------------------------------------------ photolysis
+ * The FAST-TUV output mapping of tuvDriver (ModTuvDriver2.7.f90:1302-1332)
+ * in the reference's loop order: for k = 1..kj with xref(k) /= 0, over the
+ * interior columns, jphoto(xref(k), m+1, i, j) = valJLoc(i, j, k, m) for
+ * m = 1..m1-1 and jphoto(xref(k), 1, i, j) = jphoto(xref(k), 2, i, j); then
+ * RELACS's relations over the whole array (:1363-1368), the constants as
+ * REAL literals.  valj is valJLoc(ia:iz, ja:jz, kj, kz), Fortran order;
+ * jphoto (np, m1, m2, m3).  Returns -1 on invalid arguments. */
+int ora_tuv_jphoto(int m1, int m2, int m3, int ia, int iz, int ja, int jz, int kj, int kz,
+                   const double *valj, const int32_t *xref, int np, double *jphoto)
+{
+    if (np != NP || kz < m1 - 1 || kj < 1) return -1;
+    const long ni = iz - ia + 1, nj = jz - ja + 1;
+#define JP(n, k, i, j) jphoto[(n) + (long)np * ((k) + (long)m1 * ((i) + (long)m2 * (j)))]
+    for (int k = 0; k < kj; k++) {
+        if (xref[k] == 0) continue;
+        const int n = xref[k] - 1;
+        if (n < 0 || n >= np) return -1;
+        for (int j = ja - 1; j < jz; j++)
+            for (int i = ia - 1; i < iz; i++) {
+                for (int m = 1; m <= m1 - 1; m++)
+                    JP(n, m, i, j) = valj[(i - (ia - 1)) + ni * ((j - (ja - 1)) + nj * (k + (long)kj * (m - 1)))];
+                JP(n, 0, i, j) = JP(n, 1, i, j);
+            }
+    }
+    for (int j = 0; j < m3; j++)
+        for (int i = 0; i < m2; i++)
+            for (int k = 0; k < m1; k++)
+                JP(13, k, i, j) = (double)0.962055f * JP(12, k, i, j) + (double)0.0106247f * JP(8, k, i, j);
+    for (int j = 0; j < m3; j++)
+        for (int i = 0; i < m2; i++)
+            for (int k = 0; k < m1; k++)
+                JP(15, k, i, j) = (12.0 * JP(12, k, i, j)) + (208.0 * JP(14, k, i, j));
+#undef JP
+    return 0;
+}

@@ -91,6 +91,13 @@ double ora_qssa_species(double y, double prod, double loss, double delt, int itc
 int ora_qssa_gather_cell(const brams_chem_args *a, long cell, double *y, double *temp, double *press,
                          double *volmol);
 
+/* The FAST-TUV output mapping of tuvDriver and RELACS's J relations
+ * (ModTuvDriver2.7.f90:1302-1332, 1363-1368): valj = valJLoc(ia:iz, ja:jz,
+ * kj, kz) (Fortran order), xref[kj] (1-based jphoto index, 0 unused), jphoto
+ * (np, m1, m2, m3) updated in place.  -1 on invalid arguments. */
+int ora_tuv_jphoto(int m1, int m2, int m3, int ia, int iz, int ja, int jz, int kj, int kz,
+                   const double *valj, const int32_t *xref, int np, double *jphoto);
+
 /* Per-block counts of the last step: attempts, accepted sub-steps. */
 int ora_block_counts(int64_t *attempts, int64_t *substeps, int max_nob);
 /* Per-block largest error accepted at dt_min (0: none) of the last step. */

@@ -82,6 +82,7 @@ def oracle():
         lib.ora_qssa_species.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
         lib.ora_qssa_species.restype = C.c_double
         lib.ora_qssa_gather_cell.argtypes = [C.POINTER(ChemArgs), C.c_long, _dp, _dp, _dp, _dp]
+        lib.ora_tuv_jphoto.argtypes = [C.c_int] * 9 + [_dp, _ip, C.c_int, _dp]
         _oracle = lib
     return _oracle
 
@@ -202,6 +203,20 @@ def oracle_prod_loss(rk, y):
         lib.ora_prod_loss(ptr(r), ptr(yc), ptr(p), ptr(l))
         P[c], L[c] = p, l
     return P, L
+
+
+def tuv_jphoto(grid, valj, xref, jphoto):
+    """The oracle's FAST-TUV mapping and RELACS relations: valj a Fortran
+    (ni, nj, kj, kz) float64 array, jphoto a Fortran (17, m1, m2, m3) float64
+    array (a mapped copy is returned)."""
+    v = np.asfortranarray(valj, dtype=np.float64)
+    out = np.array(jphoto, dtype=np.float64, order="F", copy=True)
+    x = np.ascontiguousarray(xref, dtype=np.int32)
+    rc = oracle().ora_tuv_jphoto(grid.m1, grid.m2, grid.m3, grid.ia, grid.iz, grid.ja, grid.jz,
+                                 v.shape[2], v.shape[3], ptr(v), ptr(x, _ip), NP, ptr(out))
+    if rc != 0:
+        raise ValueError("ora_tuv_jphoto: invalid arguments")
+    return out
 
 
 def qssa_species(y, prod, loss, delt, itchim):
