@@ -47,11 +47,19 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6
 
 
+def mech_flops():
+    """The generator's operation counts (generated/<mech>_flops.json) and the
+    species count of the process's mechanism (BRAMS_MECH, RELACS_TUV by
+    default)."""
+    from brams_amd.mechanism import default_name, load_mechanism
+    with open(os.path.join(ROOT, "brams_amd", "csrc", "generated", "%s_flops.json" % default_name())) as fh:
+        f = json.load(fh)
+    return f, load_mechanism().nspecies
+
+
 def flop_model():
     """Algorithmic FP64 FLOP of the reference algorithm, per cell."""
-    with open(os.path.join(ROOT, "brams_amd", "csrc", "generated", "relacs_tuv_flops.json")) as fh:
-        f = json.load(fh)
-    ns = 47
+    f, ns = mech_flops()
     # per attempt: diagonal update, elimination, 4 solves, f(y3), f(y4), stage
     # vectors (rhs2 2n, y3 2n, rhs3 3n, y4 3n, rhs4 4n, y1 4n, tol 3n, err 3n)
     # and the norm (2 per cell)
@@ -71,9 +79,7 @@ def ros2_flop_model():
     cell: per attempt the diagonal update, the elimination, 2 solves, f(y+k1/g)
     and the stage vectors (y+k1/g 2n, rhs2 3n, y_new 4n, error term 4n, tol
     3n, err 3n) with the norm; per accepted sub-step and per call as RODAS3."""
-    with open(os.path.join(ROOT, "brams_amd", "csrc", "generated", "relacs_tuv_flops.json")) as fh:
-        f = json.load(fh)
-    ns = 47
+    f, ns = mech_flops()
     vec = (2 + 3 + 4 + 4 + 3 + 3) * ns + 2
     f_att = ns + f["elimination"] + 2 * f["solve"] + f["fun"] + vec
     f_call, f_sub, _ = flop_model()
@@ -85,14 +91,11 @@ def qssa_flop_model(itchim):
     the rate coefficients and conversions (as RODAS3's per-call term), the
     P/L split, per species the setup (ybid, P/ybid, the exponent) and the
     tendency (10), and per species and sub-step one subtraction and one FMA."""
-    with open(os.path.join(ROOT, "brams_amd", "csrc", "generated", "relacs_tuv_flops.json")) as fh:
-        f = json.load(fh)
-    ns = 47
+    f, ns = mech_flops()
     f_call, _, _ = flop_model()
     return f_call + f["prod_loss"] + 10 * ns + 3 * ns * itchim
 
 
-QSSA_BYTES_PER_CELL = 3 * 47 * 4 + 4 * 4 + 17 * 8     # sc_p in, sc_t in and out, met, J
 # PMC-measured HBM bytes per cell of qssa_cells (read + written),
 # profiles/qssa_kernel_stats_and_traffic.md
 QSSA_TRAFFIC_PER_CELL = 881 + 289
@@ -290,16 +293,22 @@ def calibration_copy(mib, dev, log):
     del src, dst
 
 
-def bench_ros2(args):
-    """The plain FP64 ROS2 solver (CHEMISTRY=3, brams_chem_ros_dyndt_device)
-    on the headline workload: --spinup ROS2 steps from the seeded state (the
-    first is the cold step), then --steps timed steps, bracketed by
-    synchronisations.  value: cell-steps/s of the timed steps."""
+def bench_plain(args, solver="ros2"):
+    """A plain FP64 solver on the headline workload: the ROS2 solver
+    (CHEMISTRY=3, brams_chem_ros_dyndt_device), or RODAS3 (CHEMISTRY=4) for
+    a mechanism other than RELACS_TUV (whose RODAS3 line is main()'s):
+    --spinup steps from the seeded state (the first is the cold step), then
+    --steps timed steps, bracketed by synchronisations.  value: cell-steps/s
+    of the timed steps."""
     import numpy as np
     import torch
     from brams_amd import abi
     from brams_amd.chem import DeviceStep
+    from brams_amd.mechanism import load_mechanism
     from brams_amd.workload import Grid, jphoto_from_sza, make_workload, tolerances
+
+    mech = load_mechanism()
+    uname = {"ros2": "ROS2", "rodas3": "RODAS3"}[solver]
 
     def log(msg):
         print("bench: " + msg, file=sys.stderr, flush=True)
@@ -308,11 +317,11 @@ def bench_ros2(args):
     wl = make_workload(grid, seed=0, sza="split", plume_sigma=20.0, plume_factor=50.0)
     atol, rtol = tolerances(args.tol)
     K, S = args.steps, args.spinup
-    ds = DeviceStep(wl, atol, rtol, device=0, solver="ros2")
+    ds = DeviceStep(wl, atol, rtol, device=0, solver=solver)
     jph = [torch.from_numpy(np.ascontiguousarray(jphoto_from_sza(grid, wl["sza_col"] + 0.5 * t)
                                                  .ravel(order="F"))).to(ds.dev) for t in range(S + K)]
-    log("ros2: workload %dx%dx%d = %d cells, %d blocks" % (args.nx, args.ny, args.nz, grid.ncell,
-                                                           len(wl["block_end"])))
+    log("%s %s: workload %dx%dx%d = %d cells, %d blocks" % (mech.name, solver, args.nx, args.ny, args.nz,
+                                                            grid.ncell, len(wl["block_end"])))
     calibration_copy(args.copy_calib_mb, ds.dev, log)
     cold = None
     for t in range(S):
@@ -323,10 +332,10 @@ def bench_ros2(args):
             cold = {"kernel_ms": st.kernel_ms, "attempts_per_cell": st.cell_attempts / st.cells,
                     "substeps_per_cell": st.cell_substeps / st.cells,
                     "cell_steps_per_s": grid.ncell / (st.kernel_ms * 1e-3)}
-        log("ros2 spin-up step %d: %.1f ms (%d cell-attempts)" % (t, st.kernel_ms, st.cell_attempts))
+        log("%s spin-up step %d: %.1f ms (%d cell-attempts)" % (solver, t, st.kernel_ms, st.cell_attempts))
     warm_sc = ds.sc_p.clone()
     warm_dt = ds.last_dt.clone()
-    f_call, f_sub, f_att = ros2_flop_model()
+    f_call, f_sub, f_att = ros2_flop_model() if solver == "ros2" else flop_model()
     wall, main_ms, flops_main, att_main = 0.0, 0.0, 0.0, 0.0
     for t in range(K):
         ds.args.jphoto = jph[S + t].data_ptr()
@@ -340,27 +349,34 @@ def bench_ros2(args):
         share = st.main_cells / st.cells
         flops_main += (f_call * st.cells + f_sub * st.cell_substeps + f_att * st.cell_attempts) * share
         att_main += st.cell_attempts * share
-        log("ros2 step %d: %.1f ms (main kernel %.1f ms, %d cell-attempts)"
-            % (t, st.kernel_ms, st.main_ms, st.cell_attempts))
+        log("%s step %d: %.1f ms (main kernel %.1f ms, %d cell-attempts)"
+            % (solver, t, st.kernel_ms, st.main_ms, st.cell_attempts))
     n64 = sum(1 for b in wl["block_end"] if b <= 64)
-    kernel = "ros2_blocks<64,%d>" % (4 if n64 >= 4096 else 1)      # ros2_launch.hip's launch shape
+    lean = mech.name != "RELACS_TUV"        # the lean library: one wave per SIMD only
+    if solver == "ros2":
+        kernel = "ros2_blocks<64,%d>" % (4 if n64 >= 4096 and not lean else 1)   # ros2_launch.hip's shape
+    else:
+        kernel = "rodas3_blocks<64,false,false,%d>" % FORM_WPS.get(abi.tuning()[0], 0)
     achieved = flops_main / (main_ms * 1e-3) / 1e12
     line = {
-        "metric": "chem cell-steps/s, FP64 ROS2 (CHEMISTRY=3) warm rollout",
+        "metric": "chem cell-steps/s, FP64 %s (CHEMISTRY=%d) warm rollout" % (uname, 3 if solver == "ros2" else 4),
         "value": grid.ncell * K / wall, "unit": "cell-steps/s", "n_gpus": 1, "steps": K,
         "warmup": S, "ms_per_step": 1e3 * wall / K, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded Philox generator, brams_amd/workload.py)",
-        "config": {"workload": "RELACS_TUV chemistry rollout, %dx%dx%d cells, %d x 120 s ROS2 steps after "
-                               "%d ROS2 spin-up steps, SZA split day/night +0.5 deg/step, NOx/VOC plume "
-                               "sigma 20 x50, seed 0" % (args.nx, args.ny, args.nz, K, S),
-                   "path": "plain FP64 ROS2 (GPU), warm rollout", "tolerances": args.tol,
+        "config": {"workload": "%s chemistry rollout, %dx%dx%d cells, %d x 120 s %s steps after "
+                               "%d %s spin-up steps, SZA split day/night +0.5 deg/step, NOx/VOC plume "
+                               "sigma 20 x50, seed 0" % (mech.name, args.nx, args.ny, args.nz, K, uname, S, uname),
+                   "mechanism": mech.name,
+                   "path": "plain FP64 %s (GPU), warm rollout" % uname, "tolerances": args.tol,
                    "cells": grid.ncell, "blocks": len(wl["block_end"]), "fp64_cold_step": cold,
-                   "flop_model_ros2": {"per_cell_call": f_call, "per_substep": f_sub, "per_attempt": f_att}},
+                   "flop_model_" + solver: {"per_cell_call": f_call, "per_substep": f_sub, "per_attempt": f_att}},
         "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": spill_traffic(kernel, att_main / K),
-                     "kernel": kernel,
-                     "avg_ms": main_ms / K, "algorithmic_bytes": 528 * grid.ncell},
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     # spill traffic measured for RELACS_TUV's kernels only
+                     "traffic": None if lean else spill_traffic(kernel, att_main / K),
+                     "kernel": kernel, "avg_ms": main_ms / K,
+                     "algorithmic_bytes": (2 * mech.nspecies * 4 + 4 * 4 + mech.nr_photo * 8) * grid.ncell},
     }
     if not args.no_cpu:
         warm_wl = dict(wl)
@@ -368,9 +384,9 @@ def bench_ros2(args):
         warm_wl["sc_p"] = ds.species_host()
         warm_wl["jphoto"] = jphoto_from_sza(grid, wl["sza_col"] + 0.5 * S)
         warm_wl["last_accepted_dt"] = warm_dt.cpu().numpy().copy()
-        log("cpu baseline: ROS2 oracle on block samples of the warm and the cold state")
+        log("cpu baseline: %s oracle on block samples of the warm and the cold state" % uname)
         line["cpu_baseline"] = cpu_baseline(warm_wl, wl, atol, rtol, warm_every=4, cold_every=16,
-                                            solver="ros2")
+                                            solver=solver)
     print(json.dumps(line), flush=True)
 
 
@@ -384,16 +400,21 @@ def bench_qssa(args):
     import torch
     from brams_amd import abi
     from brams_amd.chem import DeviceStep
+    from brams_amd.mechanism import load_mechanism
     from brams_amd.workload import Grid, jphoto_from_sza, make_workload
 
     def log(msg):
         print("bench: " + msg, file=sys.stderr, flush=True)
 
+    mech = load_mechanism()
+    ns = mech.nspecies
+    lean = mech.name != "RELACS_TUV"
+    qbytes = 3 * ns * 4 + 4 * 4 + mech.nr_photo * 8     # sc_p in, sc_t in and out, met, J
     itchim = 75
     grid = Grid(args.nx, args.ny, args.nz, dtlt=120.0)
     wl = make_workload(grid, seed=0, sza="split", plume_sigma=20.0, plume_factor=50.0, all_ppbm=True)
     K = args.steps
-    ds = DeviceStep(wl, np.zeros(47), np.zeros(47), device=0)
+    ds = DeviceStep(wl, np.zeros(ns), np.zeros(ns), device=0)
     jph = [torch.from_numpy(np.ascontiguousarray(jphoto_from_sza(grid, wl["sza_col"] + 0.5 * t)
                                                  .ravel(order="F"))).to(ds.dev) for t in range(K)]
     log("qssa: workload %dx%dx%d = %d cells" % (args.nx, args.ny, args.nz, grid.ncell))
@@ -422,16 +443,19 @@ def bench_qssa(args):
         "warmup": args.warmup, "ms_per_step": 1e3 * wall / K, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded Philox generator, brams_amd/workload.py), every species in ppbm",
-        "config": {"workload": "RELACS_TUV chemistry, %dx%dx%d cells, %d QSSA calls of 120 s with 75 "
+        "config": {"workload": "%s chemistry, %dx%dx%d cells, %d QSSA calls of 120 s with 75 "
                                "sub-steps, SZA split day/night +0.5 deg/call, NOx/VOC plume sigma 20 x50, "
-                               "seed 0" % (args.nx, args.ny, args.nz, K),
+                               "seed 0" % (mech.name, args.nx, args.ny, args.nz, K),
+                   "mechanism": mech.name,
                    "path": "QSSA (GPU)", "cells": grid.ncell, "itchim": itchim,
-                   "flop_per_cell": flop, "algorithmic_bytes_per_cell": QSSA_BYTES_PER_CELL},
+                   "flop_per_cell": flop, "algorithmic_bytes_per_cell": qbytes},
         "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": QSSA_TRAFFIC_PER_CELL * grid.ncell,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     # traffic measured for RELACS_TUV's kernel only
+                     "traffic": None if lean else QSSA_TRAFFIC_PER_CELL * grid.ncell,
                      "kernel": "qssa_cells",
-                     "avg_ms": kms / K, "algorithmic_bytes": QSSA_BYTES_PER_CELL * grid.ncell,
-                     "hbm_frac": QSSA_BYTES_PER_CELL * grid.ncell / (kms / K * 1e-3) / 8e12},
+                     "avg_ms": kms / K, "algorithmic_bytes": qbytes * grid.ncell,
+                     "hbm_frac": qbytes * grid.ncell / (kms / K * 1e-3) / 8e12},
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -537,10 +561,20 @@ def main():
               "on one grid; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    from brams_amd.mechanism import default_name
+    if default_name() != "relacs_tuv":
+        # another mechanism (BRAMS_MECH): the solvers' plain FP64 paths; the
+        # surrogate model and the TUV line are RELACS_TUV's
+        if args.path in ("surrogate", "tuv"):
+            print("bench.py --path %s measures RELACS_TUV; with BRAMS_MECH=%s use --path fp64, ros2, qssa "
+                  "or nested" % (args.path, default_name()), file=sys.stderr)
+            sys.exit(2)
+        if args.path == "fp64":
+            return bench_plain(args, "rodas3")
     if args.path == "nested":
         return bench_nested(args)
     if args.path == "ros2":
-        return bench_ros2(args)
+        return bench_plain(args, "ros2")
     if args.path == "qssa":
         return bench_qssa(args)
     if args.path == "tuv":

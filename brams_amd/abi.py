@@ -6,10 +6,20 @@ fallback anywhere in the product path.
 import ctypes as C
 import os
 
-PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libbrams_chem.so")
+from .mechanism import default_name, load_mechanism
 
-NSPECIES, NREACT, NPHOTO = 47, 128, 17
+PKG = os.path.dirname(os.path.abspath(__file__))
+
+
+def lib_path(mech=None):
+    """libbrams_chem.so for RELACS_TUV, libbrams_chem_<mech>.so for the others."""
+    mech = mech or default_name()
+    return os.path.join(PKG, "libbrams_chem.so" if mech == "relacs_tuv" else "libbrams_chem_%s.so" % mech)
+
+
+LIB_PATH = lib_path()
+_MECH = load_mechanism()
+NSPECIES, NREACT, NPHOTO = _MECH.nspecies, _MECH.nr, _MECH.nr_photo
 SPLIT = {"SYMMETRIC": 0, "SEQUENTIAL": 1, "PARALLEL": 2}
 PHOTOJ = {"FAST-TUV": 0, "FAST-JX": 1, "LUT": 2}
 
@@ -24,7 +34,8 @@ EXPORTS = ["brams_chem_init", "brams_chem_rodas3_dyndt", "brams_chem_rodas3_dynd
            "brams_chem_qssa", "brams_chem_qssa_device", "brams_chem_tuv_jphoto_device",
            "brams_chem_alloc_index", "brams_chem_get_stats", "brams_chem_last_error",
            "brams_chem_finalize", "brams_chem_block_status", "brams_chem_block_thresholds",
-           "brams_chem_set_form", "brams_chem_get_tuning"]
+           "brams_chem_set_form", "brams_chem_get_tuning", "brams_chem_mechanism",
+           "brams_chem_dt_history_bytes", "brams_chem_dt_history_pack", "brams_chem_dt_history_unpack"]
 
 
 class ChemArgs(C.Structure):
@@ -57,6 +68,11 @@ class TuvMapArgs(C.Structure):
         ("valj", C.c_void_p), ("xref", C.POINTER(C.c_int32)), ("nr_photo", C.c_int32),
         ("jphoto", C.c_void_p), ("stream", C.c_void_p),
     ]
+
+
+class DtHistoryGrid(C.Structure):
+    """brams_dt_history_grid"""
+    _fields_ = [(k, C.c_int32) for k in ("m1", "m2", "m3", "ia", "iz", "ja", "jz", "block_size")]
 
 
 class ChemDevArgs(C.Structure):
@@ -105,8 +121,8 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise ChemError("libbrams_chem.so is not built (run __graft_entry__.build()); "
-                            "there is no CPU fallback")
+            raise ChemError("%s is not built (run __graft_entry__.build()); "
+                            "there is no CPU fallback" % os.path.basename(LIB_PATH))
         # torch ships its own HIP runtime under the soname libamdhip64.so.7.
         # Loaded first, it also serves this library (one runtime, one HSA
         # instance); loaded after /opt/rocm's copy, torch would start a
@@ -132,6 +148,14 @@ def lib():
         L.brams_chem_block_thresholds.argtypes = [_ip, C.c_int32]
         L.brams_chem_set_form.argtypes = [C.c_int32]
         L.brams_chem_get_tuning.argtypes = [_ip, C.POINTER(C.c_float), C.c_int32]
+        L.brams_chem_mechanism.argtypes = [_ip, _ip, _ip]
+        L.brams_chem_mechanism.restype = C.c_char_p
+        L.brams_chem_dt_history_bytes.argtypes = [C.c_int32]
+        L.brams_chem_dt_history_bytes.restype = C.c_int64
+        L.brams_chem_dt_history_pack.argtypes = [C.POINTER(DtHistoryGrid), _dp, C.c_int32, C.c_void_p, C.c_int64]
+        L.brams_chem_dt_history_pack.restype = C.c_int64
+        L.brams_chem_dt_history_unpack.argtypes = [C.c_void_p, C.c_int64, C.POINTER(DtHistoryGrid), _dp,
+                                                   C.c_int32]
         _lib = L
     return _lib
 
@@ -175,6 +199,13 @@ def tuning():
     n = check(lib().brams_chem_get_tuning(f.ctypes.data_as(_ip), ms.ctypes.data_as(C.POINTER(C.c_float)), 8),
               "brams_chem_get_tuning")
     return int(f[0]), ms[:n]
+
+
+def mechanism():
+    """(name, nspecies, nr, nr_photo) of the mechanism the library holds."""
+    n = [C.c_int32() for _ in range(3)]
+    name = lib().brams_chem_mechanism(*[C.byref(x) for x in n]).decode()
+    return (name,) + tuple(x.value for x in n)
 
 
 def stats():

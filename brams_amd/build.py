@@ -1,9 +1,12 @@
 """Build every native piece in-tree.
 
-  1. brams_amd/codegen/gen_mech.py  -> brams_amd/csrc/generated/relacs_tuv_gen.h
-  2. hipcc (gfx950)                 -> brams_amd/libbrams_chem.so
+  1. brams_amd/codegen/gen_mech.py  -> brams_amd/csrc/generated/<mech>_gen.h
+                                       (+ _slab.h, _flops.json) per mechanism
+  2. hipcc (gfx950)                 -> brams_amd/libbrams_chem.so (RELACS_TUV)
+                                       and libbrams_chem_<mech>.so for the
+                                       other mechanisms (lean kernel set)
   3. flang                          -> brams_amd/fortran/ shim objects + test driver
-  4. make -C oracle                 -> oracle/liboracle.so (+ oracle/_ref when the
+  4. make -C oracle                 -> oracle/liboracle*.so (+ oracle/_ref when the
                                        reference tree is present)
 """
 import concurrent.futures
@@ -15,7 +18,6 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
-LIB = os.path.join(PKG, "libbrams_chem.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
@@ -34,19 +36,39 @@ OBJ = os.path.join(PKG, "_build")
 # parallel; the slab kernels (two minutes) live in rodas3_slab.hip.
 HIP_UNITS = ["rodas3_launch.hip", "rodas3_launch_b.hip", "rodas3_grids.hip", "ros2_launch.hip",
              "rodas3_slab.hip", "qssa_launch.hip", "brams_chem.hip", "surrogate.hip"]
-_RODAS3_DEPS = ["rodas3_kernel.h", "rodas3_launch.h", "mech_families.h", "generated/relacs_tuv_gen.h"]
-UNIT_DEPS = {
-    "rodas3_launch.hip": _RODAS3_DEPS,
-    "rodas3_launch_b.hip": _RODAS3_DEPS,
-    "rodas3_grids.hip": _RODAS3_DEPS,
-    "ros2_launch.hip": _RODAS3_DEPS,
-    "qssa_launch.hip": _RODAS3_DEPS,
-    "rodas3_slab.hip": ["rodas3_slab.h", "rodas3_slab_launch.h", "rodas3_kernel.h", "mech_families.h",
-                        "generated/relacs_tuv_gen.h", "generated/relacs_tuv_slab.h"],
-}
-# parallel compiles (each needs a few GB of host memory)
-HIP_JOBS = int(os.environ.get("BRAMS_HIP_JOBS", "6"))
+# the mechanisms and their libraries: RELACS_TUV with every kernel form, the
+# others lean (BRAMS_MECH_LEAN: the per-lane kernels at one wave per SIMD,
+# no slab, list or several-grid forms), since one instantiation of RACM's
+# kernel takes about three minutes to compile
+DEFAULT_MECH = "relacs_tuv"
+MECHS = ["racm_tuv", "relacs_tuv", "relacs_mx", "cb07_tuv"]   # longest compile first
+_KERNEL_UNITS = {"rodas3_launch.hip", "rodas3_launch_b.hip", "rodas3_grids.hip", "ros2_launch.hip",
+                 "qssa_launch.hip"}
+_KERNEL_DEPS = ["rodas3_kernel.h", "rodas3_launch.h", "mech_families.h", "mechanism.h"]
+# parallel compiles, one per available CPU (at most 16; each needs a few GB
+# of host memory)
+HIP_JOBS = int(os.environ.get("BRAMS_HIP_JOBS", str(min(16, len(os.sched_getaffinity(0))))))
 REF_ROOT = os.environ.get("BRAMS_REFERENCE", "/root/reference")
+
+
+def lib_file(mech):
+    return os.path.join(PKG, "libbrams_chem.so" if mech == DEFAULT_MECH else "libbrams_chem_%s.so" % mech)
+
+
+def mech_flags(mech):
+    if mech == DEFAULT_MECH:
+        return []
+    return ['-DBRAMS_MECH_GEN="generated/%s_gen.h"' % mech,
+            '-DBRAMS_MECH_SLAB="generated/%s_slab.h"' % mech, "-DBRAMS_MECH_LEAN"]
+
+
+def unit_deps(unit, mech):
+    gen_h = ["generated/%s_gen.h" % mech]
+    if unit in _KERNEL_UNITS:
+        return _KERNEL_DEPS + gen_h
+    if unit == "rodas3_slab.hip":
+        return ["rodas3_slab.h", "rodas3_slab_launch.h"] + _KERNEL_DEPS + gen_h + ["generated/%s_slab.h" % mech]
+    return None     # every header
 
 
 def run(cmd, **kw):
@@ -55,8 +77,9 @@ def run(cmd, **kw):
 
 
 def gen():
-    run([sys.executable, os.path.join(PKG, "codegen", "gen_mech.py")])
-    run([sys.executable, os.path.join(ROOT, "oracle", "gen_tables.py")])
+    for mech in MECHS:
+        run([sys.executable, os.path.join(PKG, "codegen", "gen_mech.py"), "--mech", mech])
+        run([sys.executable, os.path.join(ROOT, "oracle", "gen_tables.py"), "--mech", mech])
 
 
 def _headers():
@@ -66,32 +89,42 @@ def _headers():
     return out
 
 
-def build_hip(force=True):
-    """Compile each unit to an object under _build/ and link the library.
+def build_hip(force=True, mechs=None):
+    """Compile each unit of each mechanism's library to an object under
+    _build/ (_build/<mech>/ for the lean ones) and link the libraries.
     force=False recompiles only units older than their source or a header."""
-    os.makedirs(OBJ, exist_ok=True)
-    objs, jobs = [], []
-    for unit in HIP_UNITS:
-        src = os.path.join(CSRC, unit)
-        obj = os.path.join(OBJ, unit.replace(".hip", ".o"))
-        deps = [os.path.join(CSRC, h) for h in UNIT_DEPS[unit]] if unit in UNIT_DEPS else _headers()
-        stale = (force or not os.path.exists(obj)
-                 or os.path.getmtime(obj) < max(os.path.getmtime(f) for f in deps + [src]))
-        if stale:
-            jobs.append((src, obj))
-        objs.append(obj)
+    mechs = mechs or MECHS
+    jobs, links = [], []
+    for mech in mechs:
+        odir = OBJ if mech == DEFAULT_MECH else os.path.join(OBJ, mech)
+        os.makedirs(odir, exist_ok=True)
+        objs = []
+        for unit in HIP_UNITS:
+            src = os.path.join(CSRC, unit)
+            obj = os.path.join(odir, unit.replace(".hip", ".o"))
+            d = unit_deps(unit, mech)
+            deps = [os.path.join(CSRC, h) for h in d] if d is not None else _headers()
+            stale = (force or not os.path.exists(obj)
+                     or os.path.getmtime(obj) < max(os.path.getmtime(f) for f in deps + [src]))
+            if stale:
+                jobs.append((src, obj, mech_flags(mech)))
+            objs.append(obj)
+        links.append((lib_file(mech), objs))
 
     def compile_unit(job):
-        src, obj = job
-        run([HIPCC] + HIP_FLAGS + ["-I", CSRC, "-c", "-o", obj + ".tmp", src])
+        src, obj, flags = job
+        run([HIPCC] + HIP_FLAGS + flags + ["-I", CSRC, "-c", "-o", obj + ".tmp", src])
         os.replace(obj + ".tmp", obj)
 
+    # the kernel units first: they take minutes, the others seconds
+    jobs.sort(key=lambda j: os.path.basename(j[0]) not in _KERNEL_UNITS)
     with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, HIP_JOBS)) as ex:
         for f in [ex.submit(compile_unit, j) for j in jobs]:
             f.result()
-    tmp = LIB + ".tmp"
-    run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
-    os.replace(tmp, LIB)
+    for lib, objs in links:
+        tmp = lib + ".tmp"
+        run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+        os.replace(tmp, lib)
 
 
 def build_fortran():

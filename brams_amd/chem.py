@@ -37,6 +37,36 @@ def alloc_index(m1, ia, iz, ja, jz, block_size=64):
     return out
 
 
+def _dth_grid(grid, block_size):
+    return abi.DtHistoryGrid(grid.m1, grid.m2, grid.m3, grid.ia, grid.iz, grid.ja, grid.jz, block_size)
+
+
+def dt_history_pack(grid, last_accepted_dt, block_size=64):
+    """The dt history record of one grid (brams_chem_dt_history_pack): the
+    bytes a history (restart) file keeps so that a restarted run takes the
+    same per-block step sizes as the uninterrupted one."""
+    dt = np.ascontiguousarray(np.asarray(last_accepted_dt, np.float64))
+    L = abi.lib()
+    n = abi.check(L.brams_chem_dt_history_bytes(len(dt)), "brams_chem_dt_history_bytes")
+    buf = C.create_string_buffer(n)
+    gr = _dth_grid(grid, block_size)
+    got = abi.check(L.brams_chem_dt_history_pack(C.byref(gr), dt.ctypes.data_as(abi._dp), len(dt), buf, n),
+                    "brams_chem_dt_history_pack")
+    return buf.raw[:got]
+
+
+def dt_history_unpack(record, grid, block_size=64):
+    """last_accepted_dt from a dt history record, checked against the grid it
+    is restored into (brams_chem_dt_history_unpack)."""
+    nob = alloc_index(grid.m1, grid.ia, grid.iz, grid.ja, grid.jz, block_size).size
+    out = np.zeros(nob, np.float64)
+    gr = _dth_grid(grid, block_size)
+    abi.check(abi.lib().brams_chem_dt_history_unpack(record, len(record), C.byref(gr),
+                                                     out.ctypes.data_as(abi._dp), nob),
+              "brams_chem_dt_history_unpack")
+    return out
+
+
 SOLVERS = {"rodas3": "brams_chem_rodas3_dyndt", "ros2": "brams_chem_ros_dyndt"}
 
 
@@ -138,7 +168,7 @@ def run_workload_qssa_host(wl, itchim=75, sc_t=None):
     sct = ([np.zeros_like(x, order="F") for x in wl["sc_p"]] if sc_t is None
            else [np.array(x, copy=True, order="F") for x in sc_t])
     rc = chem_qssa(g.m1, g.m2, g.m3, g.ia, g.iz, g.ja, g.jz, itchim, g.dtlt, wl["pp"], wl["pi0"],
-                   wl["theta"], wl["rv"], wl["jphoto"], mech.nspecies, mech.nr, 17,
+                   wl["theta"], wl["rv"], wl["jphoto"], mech.nspecies, mech.nr, mech.nr_photo,
                    [s.weight for s in mech.species], "FAST-TUV", wl["sc_p"], sct, 1004.0,
                    1004.0 / 287.0, 1.0e5)
     return sct, rc
@@ -149,7 +179,7 @@ def tuv_jphoto_device(grid, valj, kj, kz, xref, jphoto, stream=None):
     (ModTuvDriver2.7.f90:1302-1332, 1363-1368) on device memory: valj a
     float64 tensor holding valJLoc(ia:iz, ja:jz, kj, kz) in Fortran order,
     xref the TUV cross-reference (kj ints: 1-based jphoto index, 0 unused),
-    jphoto a float64 tensor holding (17, m1, m2, m3) in Fortran order,
+    jphoto a float64 tensor holding (nr_photo, m1, m2, m3) in Fortran order,
     updated in place."""
     L = abi.lib()
     x = np.ascontiguousarray(xref, dtype=np.int32)
@@ -160,7 +190,7 @@ def tuv_jphoto_device(grid, valj, kj, kz, xref, jphoto, stream=None):
     a.kj, a.kz = kj, kz
     a.valj = valj.data_ptr()
     a.xref = x.ctypes.data_as(C.POINTER(C.c_int32))
-    a.nr_photo = 17
+    a.nr_photo = load_mechanism().nr_photo
     a.jphoto = jphoto.data_ptr()
     a.stream = stream
     return abi.check(L.brams_chem_tuv_jphoto_device(C.byref(a)), "brams_chem_tuv_jphoto_device")
@@ -178,7 +208,7 @@ def run_workload_host(wl, atol, rtol, split_method="SYMMETRIC", n_dyn_chem=1, sc
     nidx = [i + 1 for i, s in enumerate(mech.species) if not s.transported]
     sct = None if sc_t is None else [np.array(x, copy=True, order="F") for x in sc_t]
     rc = chem_rodas3_dyndt(g.m1, g.m2, g.m3, g.ia, g.iz, g.ja, g.jz, g.dtlt, wl["pp"], wl["pi0"],
-                           wl["theta"], wl["rv"], mech.nspecies, mech.nr, 17,
+                           wl["theta"], wl["rv"], mech.nspecies, mech.nr, mech.nr_photo,
                            [s.weight for s in mech.species], "FAST-TUV", tidx, nidx, sc_p,
                            len(wl["block_end"]), int(max(wl["block_end"])), wl["block_end"],
                            last_dt, atol, rtol, n_dyn_chem, split_method, 1004.0, 1004.0 / 287.0,
@@ -212,7 +242,7 @@ class DeviceStep:
         self.theta = up(wl["theta"], np.float32)
         self.rv = up(wl["rv"], np.float32)
         self.jphoto = up(wl["jphoto"], np.float64)
-        self.sc_p0 = torch.stack([up(x, np.float32) for x in wl["sc_p"]])       # [47, n3d]
+        self.sc_p0 = torch.stack([up(x, np.float32) for x in wl["sc_p"]])       # [nspecies, n3d]
         self.sc_p = self.sc_p0.clone()
         self.weight = torch.tensor([s.weight for s in mech.species], dtype=torch.float32, device=dev)
         self.transported = torch.tensor([1 if s.transported else 0 for s in mech.species],
@@ -249,7 +279,7 @@ class DeviceStep:
 
     def qssa(self, itchim=75):
         """brams_chem_qssa_device on the resident state: adds every species'
-        tendency to self.sc_t ([47, n3d], zeros at first use); sc_p is not
+        tendency to self.sc_t ([nspecies, n3d], zeros at first use); sc_p is not
         changed.  Returns the number of cells that reached 1e20."""
         if getattr(self, "sc_t", None) is None:
             self.sc_t = self.torch.zeros_like(self.sc_p)

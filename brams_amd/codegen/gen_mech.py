@@ -1,8 +1,9 @@
 #!/usr/bin/env python3
-"""Generate the RELACS_TUV device functions for the RODAS3 kernels.
+"""Generate a mechanism's device functions for the chemistry kernels.
 
-Reads brams_amd/mechanism/relacs_tuv.json and writes
-brams_amd/csrc/generated/relacs_tuv_gen.h with straight-line HIP device code:
+Reads brams_amd/mechanism/<mech>.json (relacs_tuv by default; relacs_mx,
+cb07_tuv, racm_tuv with --mech) and writes
+brams_amd/csrc/generated/<mech>_gen.h with straight-line HIP device code:
 
   mech_kinetic  rk[128] from T, p, rv, J[17]       (chem_spack_kinetic.f90)
   mech_fun      f[47] = S w(rk, y)                  (rates.f90 + fexchem.f90)
@@ -22,8 +23,25 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
-JSON = os.path.join(PKG, "mechanism", "relacs_tuv.json")
-OUT = os.path.join(PKG, "csrc", "generated", "relacs_tuv_gen.h")
+MECH = "relacs_tuv"
+GEN = os.path.join(PKG, "csrc", "generated")
+
+
+def json_path():
+    return os.path.join(PKG, "mechanism", MECH + ".json")
+
+
+# J relations that tuvDriver applies after the TUV mapping, per
+# chemical_mechanism (ModTuvDriver2.7.f90:1333-1368), in the reference's
+# order: (target, expression over v[] with 0-based indices).  Constants are
+# REAL literals.
+J_RELATIONS = {
+    "RELACS": [(13, "(double)0.962055f * v[12] + (double)0.0106247f * v[8]"),
+               (15, "12.0 * v[12] + 208.0 * v[14]")],
+    "RACM": [(14, "(double)0.28f * v[8]"), (19, "(double)0.2f * v[0]"), (13, "v[12]"),
+             (15, "(v[15] + (double)8.7e-07f) / 2.0"), (22, "v[15]")],
+    "CB07": [(14, "v[13]")],
+}
 
 
 def lit(x):
@@ -35,7 +53,7 @@ def lit(x):
 
 
 def load():
-    m = json.load(open(JSON))
+    m = json.load(open(json_path()))
     ns, nr = m["nspecies"], m["nr"]
     react = [[x - 1 for x in r["reactants"]] for r in m["reactions"]]
     prod = {sp - 1: [(c, r - 1) for c, r in terms] for sp, terms in m["production"]}
@@ -121,6 +139,12 @@ def rate_expr(fam, p, r):
         return "rl_hno3_oh(T, M, %s, %s, %s, %s, %s, %s)" % tuple(lit(x) for x in p)
     if fam == "linear_m":
         return "%s * (1.0 + %s * M)" % (lit(p[0]), lit(p[1]))
+    if fam == "troe_ca":
+        return "rl_troe_ca(T, M, %s, %s, %s, %s, %s)" % tuple(lit(x) for x in p)
+    if fam == "h2o_sq":
+        return "%s * H2O * H2O" % lit(p[0])
+    if fam == "m_exp_h2o":
+        return "%s * M * exp(%s / T) * (1.0 + %s * H2O * exp(%s / T))" % tuple(lit(x) for x in p)
     raise ValueError(fam)
 
 
@@ -156,10 +180,11 @@ def main():
 
     L = []
     w = L.append
-    w("// Generated by brams_amd/codegen/gen_mech.py from brams_amd/mechanism/relacs_tuv.json.")
-    w("// RELACS_TUV: %d species, %d reactions, %d Jacobian entries, LU nnz %d." % (ns, nr, len(jt), nlu))
+    w("// Generated by brams_amd/codegen/gen_mech.py from brams_amd/mechanism/%s.json." % MECH)
+    w("// %s: %d species, %d reactions, %d Jacobian entries, LU nnz %d." % (m["name"], ns, nr, len(jt), nlu))
     w("#pragma once")
     w("#include \"mech_families.h\"")
+    w("#define MECH_NAME \"%s\"" % m["name"])
     w("#define MECH_NS %d" % ns)
     w("#define MECH_NR %d" % nr)
     w("#define MECH_NPHOT %d" % m["nr_photo"])
@@ -186,6 +211,8 @@ def main():
             kind = post[0][0]
             if kind == "mul_M_const":
                 w("  rk[%d] = (%s) * M * %s;" % (r, e, lit(post[0][1])))
+            elif kind == "mul_M":
+                w("  rk[%d] = (%s) * M;" % (r, e))
             elif kind == "mul_H2O":
                 w("  rk[%d] = (%s) * H2O;" % (r, e))
             elif kind == "mul_troe":
@@ -235,6 +262,8 @@ def main():
     # ---------------- production / loss of the QSSA split
     def dwexpr(r, s):
         rs = react[r]
+        if s not in rs:          # negative product (CB07_TUV): dratedc never assigns dw(r,s)
+            return "0.0"
         if len(rs) == 1:
             return "rk[%d]" % r
         o = rs[1] if rs[0] == s else rs[0]
@@ -401,12 +430,26 @@ def main():
     for k in ("fun", "factor", "solve"):
         w("#define MECH_FLOPS_%s %d" % (k.upper(), flops[k]))
 
-    os.makedirs(os.path.dirname(OUT), exist_ok=True)
-    with open(os.path.join(os.path.dirname(OUT), "relacs_tuv_flops.json"), "w") as fh:
+    w("")
+    # ---------------- J relations after the TUV mapping
+    rel = J_RELATIONS.get(m.get("chemical_mechanism", ""), [])
+    w("// J relations of tuvDriver for %s (ModTuvDriver2.7.f90:1333-1368), each" % m.get("chemical_mechanism"))
+    w("// product and sum rounded as the reference's x86-64 build does (no FMA)")
+    w("#define MECH_J_RELATIONS %d" % len(rel))
+    w("__device__ __forceinline__ void mech_j_relations(double* v) {")
+    w("#pragma clang fp contract(off)")
+    if not rel:
+        w("  (void)v;")
+    for t, e in rel:
+        w("  v[%d] = %s;" % (t, e))
+    w("}")
+
+    os.makedirs(GEN, exist_ok=True)
+    with open(os.path.join(GEN, MECH + "_flops.json"), "w") as fh:
         json.dump(flops, fh, indent=1)
     text = "\n".join(L) + "\n"
     emit_slab(m, ns, react, prod, jt, perm, F, slot, sentries, sidx, wexpr, sum_terms, jexpr, rate_expr)
-    out = sys.argv[1] if len(sys.argv) > 1 else OUT
+    out = os.path.join(GEN, MECH + "_gen.h")
     old = open(out).read() if os.path.exists(out) else None
     if old != text:
         with open(out, "w") as fh:
@@ -414,7 +457,8 @@ def main():
     print("generated %s: nstruct=%d nlu=%d" % (out, len(sentries), nlu))
 
 
-SLAB_OUT = os.path.join(PKG, "csrc", "generated", "relacs_tuv_slab.h")
+def slab_out():
+    return os.path.join(GEN, MECH + "_slab.h")
 RL_REGS = 120        # factorisation entries kept in registers at most
 
 
@@ -444,10 +488,10 @@ def emit_slab(m, ns, react, prod, jt, perm, F, slot, sentries, sidx, wexpr, sum_
     jorder = sorted(jt.keys(), key=lambda e: (e[1], e[0]))          # column-major
     jidx = {e: k for k, e in enumerate(jorder)}
     nlu = len(slot)
-    w("// Generated by brams_amd/codegen/gen_mech.py from brams_amd/mechanism/relacs_tuv.json.")
+    w("// Generated by brams_amd/codegen/gen_mech.py from brams_amd/mechanism/%s.json." % MECH)
     w("// Accessor-templated device functions of the slab kernel (rodas3_slab.h).")
     w("#pragma once")
-    w("#include \"relacs_tuv_gen.h\"")
+    w("#include \"%s_gen.h\"" % MECH)
     w("#define MECH_NJAC %d" % len(jorder))
     w("")
     # ---- kinetic
@@ -466,6 +510,8 @@ def emit_slab(m, ns, react, prod, jt, perm, F, slot, sentries, sidx, wexpr, sum_
             kind = post[0][0]
             if kind == "mul_M_const":
                 w("  rk[%d] = (%s) * M * %s;" % (r, e, lit(post[0][1])))
+            elif kind == "mul_M":
+                w("  rk[%d] = (%s) * M;" % (r, e))
             elif kind == "mul_H2O":
                 w("  rk[%d] = (%s) * H2O;" % (r, e))
             elif kind == "mul_troe":
@@ -709,11 +755,13 @@ def emit_slab(m, ns, react, prod, jt, perm, F, slot, sentries, sidx, wexpr, sum_
         w("  x[%d] = c%d;" % (perm[I], I))
     w("}")
     text = "\n".join(L) + "\n"
-    old = open(SLAB_OUT).read() if os.path.exists(SLAB_OUT) else None
+    old = open(slab_out()).read() if os.path.exists(slab_out()) else None
     if old != text:
-        with open(SLAB_OUT, "w") as fh:
+        with open(slab_out(), "w") as fh:
             fh.write(text)
 
 
 if __name__ == "__main__":
+    if len(sys.argv) > 2 and sys.argv[1] == "--mech":
+        MECH = sys.argv[2].lower()
     main()

@@ -11,6 +11,7 @@
 //      more than 64 cells and wave64 workgroups for the rest.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -77,6 +78,8 @@ struct DevBuf {
 constexpr int kFormPerLane = brams::kSlabVariants;      // first per-lane form
 constexpr int kPerLaneWps[3] = {1, 2, 4};
 constexpr int kForms = brams::kSlabVariants + 3;
+// per-lane forms the library holds (a lean build: 1 wave per SIMD only)
+constexpr int kPerLaneBuilt = brams::kLeanBuild ? 1 : 3;
 // List (fallback) launches: the per-lane kernel at 1 or 2 waves per SIMD,
 // by the launch's block count.  Measured on the 1.6M-cell rollout (ms per
 // launch, 1 / 2 waves per SIMD): 19 400 blocks 246 / 190; the 2000..6000
@@ -142,6 +145,14 @@ struct State {
   brams_chem_stats stats{};
 };
 State g;
+
+// the host entries' per-species constants in g.consts: weight f32,
+// transport flag i32, atol f64, rtol f64, each region 256-byte aligned
+constexpr size_t kConstRound(size_t x) { return (x + 255) / 256 * 256; }
+constexpr size_t kConstOffT = kConstRound(sizeof(float) * MECH_NS);
+constexpr size_t kConstOffA = kConstOffT + kConstRound(sizeof(int) * MECH_NS);
+constexpr size_t kConstOffR = kConstOffA + kConstRound(sizeof(double) * MECH_NS);
+constexpr size_t kConstBytes = kConstOffR + kConstRound(sizeof(double) * MECH_NS);
 
 // Bookkeeping of brams_chem_rodas3_dyndt_device_noalloc (block starts, mask
 // words, per-block counters, failure count) lives in one file-scope device
@@ -371,7 +382,7 @@ int autotune(brams::StepParams p, int first64, int nob, hipStream_t st) {
   q.last_dt = dt1;
   const int f0 = kSlabEnabled ? 0 : kFormPerLane;
   int best = f0;
-  for (int f = f0; f < kForms; ++f) {
+  for (int f = f0; f < kFormPerLane + kPerLaneBuilt; ++f) {
     HIP_TRY(hipMemcpyAsync(sc1, sc0, nsc, hipMemcpyDeviceToDevice, st));
     if (to_t) HIP_TRY(hipMemcpyAsync(st1, st0, nsc, hipMemcpyDeviceToDevice, st));
     HIP_TRY(hipMemcpyAsync(dt1, dt0, sizeof(double) * nob, hipMemcpyDeviceToDevice, st));
@@ -553,6 +564,15 @@ int check_grid(int m1, int m2, int m3, int ia, int iz, int ja, int jz, int photo
   return 0;
 }
 
+// The surrogate's feature and target layout (include/brams_surrogate.h) is
+// RELACS_TUV's: a build for another mechanism has no surrogate step.
+int check_surrogate_mechanism() {
+  if (brams::kSurNin != BRAMS_SUR_NIN || brams::kSurNout != BRAMS_SUR_NOUT || !brams::kSurrogateBuilt)
+    return fail(-4, "the surrogate step exists for RELACS_TUV only (%d features, %d species; this build has "
+                "%d, %d)", BRAMS_SUR_NIN, BRAMS_SUR_NOUT, brams::kSurNin, brams::kSurNout);
+  return 0;
+}
+
 int check_common(int m1, int m2, int m3, int ia, int iz, int ja, int jz, int ndyn, int split,
                  int photoj, float dtlt) {
   int rc = check_grid(m1, m2, m3, ia, iz, ja, jz, photoj, dtlt);
@@ -604,6 +624,130 @@ int brams_chem_alloc_index(int32_t block_size, int32_t m1, int32_t ia, int32_t i
     if (nob > max_nob) return fail(-2, "max_nob %d < nob %d", max_nob, nob);
     for (int b = 0; b < nob; ++b) block_end[b] = sz[b];
   }
+  return nob;
+}
+
+const char* brams_chem_mechanism(int32_t* nspecies, int32_t* nr, int32_t* nr_photo) {
+  if (nspecies) *nspecies = MECH_NS;
+  if (nr) *nr = MECH_NR;
+  if (nr_photo) *nr_photo = MECH_NPHOT;
+  return MECH_NAME;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// dt history record (host code only).  Layout, little-endian:
+//   0  char[8]  "BRAMSDTH"          8  uint32 version (1)   12 uint32 header bytes (72)
+//   16 char[16] mechanism name, NUL-padded
+//   32 int32[8] m1 m2 m3 ia iz ja jz block_size
+//   64 int32    nob                 68 int32 0
+//   72 double[nob] last_accepted_dt
+//   72+8*nob uint64 FNV-1a (64-bit) of every byte before it
+namespace {
+
+constexpr char kDthMagic[8] = {'B', 'R', 'A', 'M', 'S', 'D', 'T', 'H'};
+constexpr uint32_t kDthVersion = 1;
+constexpr int64_t kDthHeader = 72;
+
+uint64_t fnv1a64(const unsigned char* p, int64_t n) {
+  uint64_t h = 0xcbf29ce484222325ull;
+  for (int64_t i = 0; i < n; ++i) {
+    h ^= p[i];
+    h *= 0x100000001b3ull;
+  }
+  return h;
+}
+
+// the partition's block count of a grid, or <0 (with the error set)
+int dth_nob(const brams_dt_history_grid* gr) {
+  if (!gr) return fail(-2, "null grid");
+  if (gr->m1 < 3 || gr->ia < 1 || gr->iz < gr->ia || gr->iz > gr->m2 || gr->ja < 1 || gr->jz < gr->ja ||
+      gr->jz > gr->m3 || gr->block_size < 1)
+    return fail(-2, "bad grid (m1 %d m2 %d m3 %d ia..iz %d..%d ja..jz %d..%d block_size %d)", gr->m1, gr->m2,
+                gr->m3, gr->ia, gr->iz, gr->ja, gr->jz, gr->block_size);
+  std::vector<int> sz;
+  return partition(gr->block_size, (long long)(gr->m1 - 2) * (gr->iz - gr->ia + 1) * (gr->jz - gr->ja + 1), sz);
+}
+
+bool dt_ok(double v) { return std::isfinite(v) && v > 0.0; }
+
+}  // namespace
+
+extern "C" {
+
+int64_t brams_chem_dt_history_bytes(int32_t nob) {
+  if (nob < 1) return fail(-2, "nob %d < 1", nob);
+  return kDthHeader + 8 * (int64_t)nob + 8;
+}
+
+int64_t brams_chem_dt_history_pack(const brams_dt_history_grid* grid, const double* last_accepted_dt,
+                                   int32_t nob, void* buf, int64_t cap) {
+  const int want = dth_nob(grid);
+  if (want < 0) return want;
+  if (nob != want) return fail(-8, "nob %d is not the grid's block count %d", nob, want);
+  if (!last_accepted_dt || !buf) return fail(-2, "null pointer");
+  const int64_t bytes = brams_chem_dt_history_bytes(nob);
+  if (cap < bytes) return fail(-2, "record needs %lld bytes, buffer holds %lld", (long long)bytes, (long long)cap);
+  for (int b = 0; b < nob; ++b)
+    if (!dt_ok(last_accepted_dt[b])) return fail(-7, "block %d: dt %g is not finite and positive", b + 1,
+                                                 last_accepted_dt[b]);
+  unsigned char* o = static_cast<unsigned char*>(buf);
+  std::memset(o, 0, kDthHeader);
+  std::memcpy(o, kDthMagic, 8);
+  const uint32_t ver = kDthVersion, hdr = (uint32_t)kDthHeader;
+  std::memcpy(o + 8, &ver, 4);
+  std::memcpy(o + 12, &hdr, 4);
+  std::strncpy(reinterpret_cast<char*>(o + 16), MECH_NAME, 15);
+  const int32_t gv[8] = {grid->m1, grid->m2, grid->m3, grid->ia, grid->iz, grid->ja, grid->jz, grid->block_size};
+  std::memcpy(o + 32, gv, sizeof gv);
+  std::memcpy(o + 64, &nob, 4);
+  std::memcpy(o + kDthHeader, last_accepted_dt, 8 * (size_t)nob);
+  const uint64_t h = fnv1a64(o, bytes - 8);
+  std::memcpy(o + bytes - 8, &h, 8);
+  return bytes;
+}
+
+int brams_chem_dt_history_unpack(const void* buf, int64_t nbytes, const brams_dt_history_grid* grid,
+                                 double* last_accepted_dt, int32_t max_nob) {
+  const int want = dth_nob(grid);
+  if (want < 0) return want;
+  if (!buf || !last_accepted_dt) return fail(-2, "null pointer");
+  const unsigned char* r = static_cast<const unsigned char*>(buf);
+  if (nbytes < kDthHeader + 16 || std::memcmp(r, kDthMagic, 8) != 0)
+    return fail(-7, "not a dt history record");
+  uint32_t ver, hdr;
+  int32_t nob;
+  std::memcpy(&ver, r + 8, 4);
+  std::memcpy(&hdr, r + 12, 4);
+  std::memcpy(&nob, r + 64, 4);
+  if (ver != kDthVersion || hdr != (uint32_t)kDthHeader)
+    return fail(-7, "dt history record version %u (header %u bytes); this library reads version %u", ver, hdr,
+                kDthVersion);
+  if (nob < 1 || nbytes != kDthHeader + 8 * (int64_t)nob + 8)
+    return fail(-7, "dt history record of %lld bytes does not hold %d blocks", (long long)nbytes, nob);
+  uint64_t h;
+  std::memcpy(&h, r + nbytes - 8, 8);
+  if (h != fnv1a64(r, nbytes - 8)) return fail(-7, "dt history record checksum mismatch (corrupt record)");
+  char mech[17] = {};
+  std::memcpy(mech, r + 16, 16);
+  if (std::strcmp(mech, MECH_NAME) != 0)
+    return fail(-8, "dt history record of mechanism %s restored into a %s library", mech, MECH_NAME);
+  int32_t gv[8];
+  std::memcpy(gv, r + 32, sizeof gv);
+  const int32_t ex[8] = {grid->m1, grid->m2, grid->m3, grid->ia, grid->iz, grid->ja, grid->jz, grid->block_size};
+  if (std::memcmp(gv, ex, sizeof gv) != 0)
+    return fail(-8, "dt history record of grid %dx%dx%d (%d..%d, %d..%d, blocks of %d) restored into grid "
+                "%dx%dx%d (%d..%d, %d..%d, blocks of %d)", gv[0], gv[1], gv[2], gv[3], gv[4], gv[5], gv[6], gv[7],
+                ex[0], ex[1], ex[2], ex[3], ex[4], ex[5], ex[6], ex[7]);
+  if (nob != want) return fail(-8, "record holds %d blocks, the grid has %d", nob, want);
+  if (max_nob < nob) return fail(-2, "max_nob %d < nob %d", max_nob, nob);
+  for (int b = 0; b < nob; ++b) {
+    double v;
+    std::memcpy(&v, r + kDthHeader + 8 * (int64_t)b, 8);
+    if (!dt_ok(v)) return fail(-7, "record block %d: dt %g is not finite and positive", b + 1, v);
+  }
+  std::memcpy(last_accepted_dt, r + kDthHeader, 8 * (size_t)nob);
   return nob;
 }
 
@@ -818,13 +962,33 @@ int brams_chem_rodas3_dyndt_device_grids(const brams_chem_dev_args* a, int32_t n
   HIP_TRY(hipMemcpyAsync(g.multi_ps.p, ps.data(), sizeof(brams::StepParams) * ngrids, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(g.multi_work.p, work.data(), sizeof(int) * nw, hipMemcpyHostToDevice, st));
   HIP_TRY(hipEventRecord(g.ev0, st));
-  HIP_TRY(brams::launch_rodas3_grids(128, true, 1, (unsigned)w1.size(), dps, dw, st));
-  HIP_TRY(hipEventRecord(g.evb, st));
-  HIP_TRY(brams::launch_rodas3_grids(128, false, 1, (unsigned)w128.size(), dps, dw + w1.size(), st));
-  HIP_TRY(hipEventRecord(g.evm, st));
-  const int wps = w64.size() >= (size_t)brams::kRodas3ManyBlocks ? brams::kRodas3ManyWaves : 1;
-  HIP_TRY(brams::launch_rodas3_grids(64, false, wps, (unsigned)w64.size(), dps, dw + w1.size() + w128.size(),
-                                     st));
+  if (brams::kLeanBuild) {
+    // a lean library has no rodas3_grids kernels: the same three phases, one
+    // launch per grid of that grid's kernels (blocks are independent, so the
+    // results are the same; the 64-cell blocks run in block order)
+    std::vector<int> nbig(ngrids, 0);
+    for (int i = 0; i < ngrids; ++i) {
+      const std::vector<int>& sz = gss[i]->sizes;
+      while (nbig[i] < (int)sz.size() && sz[nbig[i]] > 64) ++nbig[i];
+      HIP_TRY(brams::launch_rodas3(sz[0] > 64 ? 128 : 64, true, 1, ps[i], 0, st));
+    }
+    HIP_TRY(hipEventRecord(g.evb, st));
+    for (int i = 0; i < ngrids; ++i)
+      if (nbig[i] > 1) HIP_TRY(brams::launch_rodas3(128, false, (unsigned)(nbig[i] - 1), ps[i], 1, st));
+    HIP_TRY(hipEventRecord(g.evm, st));
+    for (int i = 0; i < ngrids; ++i) {
+      const int first64 = std::max(nbig[i], 1), nob = (int)gss[i]->sizes.size();
+      if (nob > first64) HIP_TRY(brams::launch_rodas3(64, false, (unsigned)(nob - first64), ps[i], first64, st, 1));
+    }
+  } else {
+    HIP_TRY(brams::launch_rodas3_grids(128, true, 1, (unsigned)w1.size(), dps, dw, st));
+    HIP_TRY(hipEventRecord(g.evb, st));
+    HIP_TRY(brams::launch_rodas3_grids(128, false, 1, (unsigned)w128.size(), dps, dw + w1.size(), st));
+    HIP_TRY(hipEventRecord(g.evm, st));
+    const int wps = w64.size() >= (size_t)brams::kRodas3ManyBlocks ? brams::kRodas3ManyWaves : 1;
+    HIP_TRY(brams::launch_rodas3_grids(64, false, wps, (unsigned)w64.size(), dps,
+                                       dw + w1.size() + w128.size(), st));
+  }
   HIP_TRY(hipEventRecord(g.ev1, st));
   // per-grid counters, then their sums (mask size: the first grid's)
   brams_chem_stats tot{};
@@ -864,7 +1028,8 @@ int brams_chem_rodas3_dyndt_device_grids(const brams_chem_dev_args* a, int32_t n
 int brams_sur_features_raw(const brams_chem_dev_args* a, float* feat, double floor_c) {
   brams::StepParams p;
   long long ncell = 0;
-  int rc = prepare_device(a, p, ncell);
+  int rc = check_surrogate_mechanism();
+  if (rc || (rc = prepare_device(a, p, ncell))) return rc;
   if (rc) return rc;
   hipStream_t st = (hipStream_t)a->stream;
   hipLaunchKernelGGL(brams::sur_raw_features, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, st,
@@ -878,6 +1043,7 @@ int brams_sur_targets_raw(const brams_chem_dev_args* a, const float* sc_p_before
                           double floor_c) {
   brams::StepParams p;
   long long ncell = 0;
+  if (int rc0 = check_surrogate_mechanism()) return rc0;
   int rc = prepare_device(a, p, ncell);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)a->stream;
@@ -919,6 +1085,7 @@ int brams_sur_normalize(const float* raw, int64_t n, int ncol, const float* mu, 
 
 int brams_sur_step(const brams_chem_dev_args* a, const brams_sur_model* m, brams_sur_stats* out) {
   if (!a) return fail(-2, "null args");
+  if (int rc0 = check_surrogate_mechanism()) return rc0;
   // the surrogate writes accepted cells straight into sc_p: the SYMMETRIC /
   // SEQUENTIAL form of the write-back only (rodas3:774-798)
   if (a->split_method == BRAMS_SPLIT_PARALLEL)
@@ -1152,9 +1319,9 @@ int host_step(const brams_chem_args* a, int solver) {
   HIP_TRY(g.met.ensure(fsz * 4));
   HIP_TRY(g.jphoto.ensure(sizeof(double) * MECH_NPHOT * n3d));
   HIP_TRY(g.last_dt.ensure(sizeof(double) * nob));
-  // consts: weight[47] f32 | transported[47] i32 | atol[47] f64 | rtol[47] f64
-  const size_t coff_w = 0, coff_t = 256, coff_a = 512, coff_r = 1024;
-  HIP_TRY(g.consts.ensure(1536));
+  // consts: weight[MECH_NS] f32 | transported[MECH_NS] i32 | atol[MECH_NS] f64 | rtol[MECH_NS] f64
+  const size_t coff_w = 0, coff_t = kConstOffT, coff_a = kConstOffA, coff_r = kConstOffR;
+  HIP_TRY(g.consts.ensure(kConstBytes));
   if (need_t) HIP_TRY(g.sc_t.ensure(fsz * MECH_NS));
   if (need_td) HIP_TRY(g.sc_t_dyn.ensure(fsz * MECH_NS));
   hipStream_t st = nullptr;
@@ -1177,7 +1344,7 @@ int host_step(const brams_chem_args* a, int solver) {
                          hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(g.last_dt.p, a->last_accepted_dt, sizeof(double) * nob,
                          hipMemcpyHostToDevice, st));
-  char cbuf[1536];
+  char cbuf[kConstBytes];
   std::memcpy(cbuf + coff_w, a->weight, sizeof(float) * MECH_NS);
   std::memcpy(cbuf + coff_t, flags, sizeof(int) * MECH_NS);
   std::memcpy(cbuf + coff_a, a->atol, sizeof(double) * MECH_NS);
@@ -1284,7 +1451,7 @@ int brams_chem_qssa(const brams_chem_args* a, int32_t itchim) {
   HIP_TRY(g.sc_t.ensure(fsz * MECH_NS));
   HIP_TRY(g.met.ensure(fsz * 4));
   HIP_TRY(g.jphoto.ensure(sizeof(double) * MECH_NPHOT * n3d));
-  HIP_TRY(g.consts.ensure(1536));
+  HIP_TRY(g.consts.ensure(kConstBytes));
   hipStream_t st = nullptr;
   char* sc = g.sc_p.as<char>();
   char* tc = g.sc_t.as<char>();
@@ -1432,6 +1599,8 @@ int brams_chem_set_form(int32_t form) {
   if (form < -1 || form >= kForms) return fail(-2, "kernel form %d out of range (-1..%d)", form, kForms - 1);
   if (!kSlabEnabled && form >= 0 && form < kFormPerLane)
     return fail(-6, "kernel form %d (slab) is switched off in this build", form);
+  if (form >= kFormPerLane + kPerLaneBuilt)
+    return fail(-6, "kernel form %d is not built into this %s library (lean build)", form, MECH_NAME);
   g.force_form = form;
   return 0;
 }

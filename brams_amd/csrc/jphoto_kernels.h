@@ -1,5 +1,6 @@
 // Photolysis rates on the device: the FAST-TUV output mapping of tuvDriver
-// (ModTuvDriver2.7.f90:1302-1332) and RELACS's J relations (:1363-1368).
+// (ModTuvDriver2.7.f90:1302-1332) and the mechanism's J relations
+// (:1333-1368; RELACS: J14 and J16, mech_j_relations of the generated code).
 //
 // tuvDriver gathers TUV's j-values into valJLoc(ia:iz, ja:jz, kj, kz) (double,
 // :1263) and writes, for every TUV reaction k with xRef(k) /= 0 in ascending
@@ -7,7 +8,8 @@
 // jphoto(xRef(k), 1, i, j) = jphoto(xRef(k), 2, i, j), over the interior
 // columns.  RELACS then sets, at every point of the array,
 //   J14 = 0.962055 J13 + 0.0106247 J9      J16 = 12 J13 + 208 J15
-// with the constants as REAL literals, without FMA contraction.
+// with the constants as REAL literals, without FMA contraction (other
+// mechanisms have their own relations, or none).
 //
 // The host resolves xRef into src[n]: the last k that writes jphoto index n
 // (later reactions overwrite earlier ones), -1 where none does (that entry
@@ -17,7 +19,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "generated/relacs_tuv_gen.h"
+#include "mechanism.h"
 
 namespace brams {
 
@@ -28,16 +30,6 @@ struct TuvMapParams {
   double* jphoto;                // (MECH_NPHOT, m1, m2, m3)
   int src[MECH_NPHOT];           // TUV reaction (0-based) that writes index n, -1: none
 };
-
-// RELACS (ModTuvDriver2.7.f90:1363-1368): J14 and J16 (1-based) from J9,
-// J13, J15 of the same point, each product and sum rounded separately (no
-// FMA contraction, as the reference's x86-64 build computes them).
-__device__ __forceinline__ void relacs_j_relations(double* v) {
-#pragma clang fp contract(off)
-  const double c1 = (double)0.962055f, c2 = (double)0.0106247f;
-  v[13] = c1 * v[12] + c2 * v[8];
-  v[15] = 12.0 * v[12] + 208.0 * v[14];
-}
 
 // Workgroups are dealt to the eight XCDs in turn (workgroup w on XCD w % 8),
 // each with its own L2.  Workgroup w takes tile (w % 8) * chunk + w / 8, so
@@ -67,12 +59,13 @@ __global__ __launch_bounds__(256) void tuv_map_interior(TuvMapParams p, long lon
     v[n] = k >= 0 ? p.valj[ii + (long long)ni * (jj + (long long)nj * (k + (long long)p.kj * mz))]
                   : p.jphoto[n + (long long)MECH_NPHOT * o];
   }
-  relacs_j_relations(v);
+  mech_j_relations(v);
 #pragma unroll
   for (int n = 0; n < MECH_NPHOT; ++n) p.jphoto[n + (long long)MECH_NPHOT * o] = v[n];
 }
 
-// points outside the interior columns: the relations alone
+// points outside the interior columns: the relations alone (the entries they
+// do not set are written back unchanged)
 __global__ __launch_bounds__(256) void jphoto_relations_outside(TuvMapParams p, long long n3d) {
   const long long o = (long long)blockIdx.x * 256 + threadIdx.x;
   if (o >= n3d) return;
@@ -83,9 +76,9 @@ __global__ __launch_bounds__(256) void jphoto_relations_outside(TuvMapParams p, 
   double v[MECH_NPHOT];
 #pragma unroll
   for (int n = 0; n < MECH_NPHOT; ++n) v[n] = jp[n];
-  relacs_j_relations(v);
-  jp[13] = v[13];
-  jp[15] = v[15];
+  mech_j_relations(v);
+#pragma unroll
+  for (int n = 0; n < MECH_NPHOT; ++n) jp[n] = v[n];
 }
 
 }  // namespace brams

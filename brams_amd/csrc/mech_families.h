@@ -1,5 +1,6 @@
-// Rate-law families of the RELACS_TUV mechanism (chem_spack_kinetic.f90),
-// device side.  Each keeps the reference's evaluation order.
+// Rate-law families of the mechanisms' kinetics (chem_spack_kinetic.f90 of
+// RELACS_TUV, RELACS_MX, RACM_TUV, CB07_TUV), device side.  Each keeps the
+// reference's evaluation order.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -31,4 +32,17 @@ __device__ __forceinline__ double rl_hno3_oh(double T, double M, double c1, doub
   const double rapk = c2 * exp(b2 / T);
   const double fac = c3 * exp(b3 / T) * M;
   return effko + fac / (1.0 + fac / rapk);
+}
+
+// fall-off of the chemical-activation form, CB07_TUV/chem_spack_kinetic.f90:231-234:
+// Effko = k0 (T/300)^n, Rapk = kinf (T/300)^m,
+// k = Effko/(1 + Effko/(Rapk/M)) * Fc^(1 + (log10(Effko/(Rapk/M))^2)^-1), the
+// exponent as the reference's parentheses group it (not the usual
+// 1/(1 + log10^2)), with its real powers
+__device__ __forceinline__ double rl_troe_ca(double T, double M, double k0, double n,
+                                             double kinf, double m, double fc) {
+  const double effko = k0 * pow(T / 300.0, n);
+  const double rapk = kinf * pow(T / 300.0, m);
+  const double x = log10(effko / (rapk / M));
+  return (effko / (1.0 + effko / (rapk / M))) * pow(fc, 1.0 + pow(pow(x, 2.0), -1.0));
 }

@@ -12,6 +12,11 @@ hipError_t launch_rodas3_grids(int wg, bool dyn, int wps, unsigned nwork, const 
                                const int* work, hipStream_t st) {
   if (nwork == 0) return hipSuccess;
   if (!grids || !work) return hipErrorInvalidValue;
+#ifdef BRAMS_MECH_LEAN
+  // a lean library (rodas3_launch.h) builds no rodas3_grids kernel
+  (void)wg, (void)dyn, (void)wps, (void)st;
+  return hipErrorInvalidValue;
+#else
   if (wg == 128 && dyn)
     hipLaunchKernelGGL((rodas3_grids<128, true, 1>), dim3(nwork), dim3(128), 0, st, grids, work);
   else if (wg == 128)
@@ -23,6 +28,7 @@ hipError_t launch_rodas3_grids(int wg, bool dyn, int wps, unsigned nwork, const 
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
+#endif
 }
 
 }  // namespace brams

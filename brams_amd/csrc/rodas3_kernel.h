@@ -16,7 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "generated/relacs_tuv_gen.h"
+#include "mechanism.h"
 
 namespace brams {
 

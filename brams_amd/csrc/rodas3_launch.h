@@ -8,6 +8,19 @@ namespace brams {
 
 struct StepParams;
 
+// Kernel forms built into the library.  A library built with
+// BRAMS_MECH_LEAN (the mechanisms other than RELACS_TUV) instantiates the
+// per-lane RODAS3 and ROS2 kernels at one wave per SIMD only, for 64 and
+// 128 lanes, and neither the list (surrogate fallback), the several-grid
+// (rodas3_grids; the entry launches grid by grid instead) nor the slab
+// kernels: each instantiation of a large mechanism takes minutes to compile.
+// The launchers below return hipErrorInvalidValue for a form not built.
+#ifdef BRAMS_MECH_LEAN
+constexpr bool kLeanBuild = true;
+#else
+constexpr bool kLeanBuild = false;
+#endif
+
 // nblocks workgroups of wg (64 or 128) lanes over blocks block0.. of p;
 // dyn: block-1 form (mask recomputed on every attempt and published).
 // wps: waves per SIMD of the 64-lane form (1, 2 or 4; 0: by block count).

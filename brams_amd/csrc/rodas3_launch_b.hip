@@ -19,13 +19,22 @@ hipError_t launch_rodas3_128(bool dyn, unsigned nblocks, const StepParams& p, in
 
 hipError_t launch_rodas3_64w2(unsigned nblocks, const StepParams& p, int block0, hipStream_t st) {
   if (nblocks == 0) return hipSuccess;
+#ifdef BRAMS_MECH_LEAN
+  (void)p, (void)block0, (void)st;
+  return hipErrorInvalidValue;
+#else
   hipLaunchKernelGGL((rodas3_blocks<64, false, false, 2>), dim3(nblocks), dim3(64), 0, st, p, block0);
   return hipGetLastError();
+#endif
 }
 
 hipError_t launch_rodas3_list(unsigned nblocks, const StepParams& p, hipStream_t st, int wps) {
   if (nblocks == 0) return hipSuccess;
   if (!p.cell_list) return hipErrorInvalidValue;
+#ifdef BRAMS_MECH_LEAN
+  (void)st, (void)wps;
+  return hipErrorInvalidValue;
+#else
   if (wps == 2)
     hipLaunchKernelGGL((rodas3_blocks<64, false, true, 2>), dim3(nblocks), dim3(64), 0, st, p, 0);
   else if (wps == 1)
@@ -33,6 +42,7 @@ hipError_t launch_rodas3_list(unsigned nblocks, const StepParams& p, hipStream_t
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
+#endif
 }
 
 }  // namespace brams

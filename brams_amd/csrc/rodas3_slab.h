@@ -32,7 +32,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "generated/relacs_tuv_slab.h"
+#include "mechanism.h"
+#include BRAMS_MECH_SLAB
 #include "rodas3_kernel.h"
 #include "rodas3_slab_launch.h"
 

@@ -8,6 +8,15 @@
 
 namespace brams {
 
+#ifdef BRAMS_MECH_LEAN
+// a lean library (rodas3_launch.h) builds no slab kernel
+int slab_variant_nlds(int v) { return (v >= 0 && v < kSlabVariants) ? 0 : -1; }
+int slab_resident_per_cu(int) { return 0; }
+hipError_t launch_rodas3_slab(int, unsigned, const StepParams&, const QueueParams&, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+#else
+
 namespace {
 
 // LU slots held in LDS per partition variant.  With 24 KB of cell state in
@@ -61,5 +70,7 @@ hipError_t launch_rodas3_slab(int variant, unsigned nslots, const StepParams& p,
     default: return hipErrorInvalidValue;
   }
 }
+
+#endif  // BRAMS_MECH_LEAN
 
 }  // namespace brams

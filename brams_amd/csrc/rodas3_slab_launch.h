@@ -2,7 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "generated/relacs_tuv_slab.h"
+#include "mechanism.h"
+#include BRAMS_MECH_SLAB
 
 namespace brams {
 

@@ -15,7 +15,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "generated/relacs_tuv_gen.h"
+#include "mechanism.h"
 
 namespace brams {
 

@@ -6,9 +6,10 @@
 // arrays with the unit conversions of chem_spack_rodas3_dyndt.f90:226-321 and
 // 713-798 (SYMMETRIC/SEQUENTIAL write-back).
 //
-// Features of one cell (68): log10 of the 47 concentrations (molec/cm^3,
-// floored), T, ln p, log10 rv, the 17 photolysis rates, log10 of the step.
-// Target (47): log10 increment of each concentration over the step.
+// Features of one cell (68 for RELACS_TUV): log10 of the MECH_NS
+// concentrations (molec/cm^3, floored), T, ln p, log10 rv, the MECH_NPHOT
+// photolysis rates, log10 of the step.  Target (MECH_NS): log10 increment of
+// each concentration over the step.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
@@ -18,10 +19,13 @@
 
 namespace brams {
 
-constexpr int kSurNin = 68;
+constexpr int kSurNin = MECH_NS + 3 + MECH_NPHOT + 1;
 constexpr int kSurNinPad = 128;
-constexpr int kSurNout = 47;
+constexpr int kSurNout = MECH_NS;
+// member outputs are padded to 64 columns; a mechanism with more species has
+// no surrogate (the step refuses every model, whose sizes cannot match)
 constexpr int kSurNoutPad = 64;
+constexpr bool kSurrogateBuilt = kSurNout <= kSurNoutPad && kSurNin <= kSurNinPad;
 
 struct CellState {
   long long o;        // offset in the (k,i,j) arrays
@@ -54,18 +58,18 @@ __device__ __forceinline__ double cell_conc(const StepParams& p, const float* sc
   return x > 0.0 ? x : 0.0;
 }
 
-// raw (unnormalised) features, one thread per cell: feat[g][68]
+// raw (unnormalised) features, one thread per cell: feat[g][kSurNin]
 __global__ void sur_raw_features(StepParams p, long long ncell, double floor_c, float* feat) {
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ncell) return;
   const CellState c = cell_state(p, g);
   float* f = feat + g * kSurNin;
   for (int s = 0; s < MECH_NS; ++s) f[s] = (float)log10(fmax(cell_conc(p, p.sc_p, c, s), floor_c));
-  f[47] = (float)c.temp;
-  f[48] = (float)log(c.press);
-  f[49] = (float)log10(fmax(c.vapp, 1e-12));
-  for (int n = 0; n < MECH_NPHOT; ++n) f[50 + n] = (float)p.jphoto[n + (long long)MECH_NPHOT * c.o];
-  f[67] = (float)log10((double)(p.dtlt * (float)p.ndyn));
+  f[MECH_NS] = (float)c.temp;
+  f[MECH_NS + 1] = (float)log(c.press);
+  f[MECH_NS + 2] = (float)log10(fmax(c.vapp, 1e-12));
+  for (int n = 0; n < MECH_NPHOT; ++n) f[MECH_NS + 3 + n] = (float)p.jphoto[n + (long long)MECH_NPHOT * c.o];
+  f[kSurNin - 1] = (float)log10((double)(p.dtlt * (float)p.ndyn));
 }
 
 // raw targets: log10 increment between two states of the same cells
@@ -136,12 +140,12 @@ __global__ __launch_bounds__(256) void sur_features(StepParams p, long long ncel
   float f[kSurNin];
 #pragma unroll
   for (int s = 0; s < MECH_NS; ++s) f[s] = (float)log10(fmax(cell_conc(p, p.sc_p, c, s), floor_c));
-  f[47] = (float)c.temp;
-  f[48] = (float)log(c.press);
-  f[49] = (float)log10(fmax(c.vapp, 1e-12));
+  f[MECH_NS] = (float)c.temp;
+  f[MECH_NS + 1] = (float)log(c.press);
+  f[MECH_NS + 2] = (float)log10(fmax(c.vapp, 1e-12));
 #pragma unroll
-  for (int n = 0; n < MECH_NPHOT; ++n) f[50 + n] = (float)p.jphoto[n + (long long)MECH_NPHOT * c.o];
-  f[67] = (float)log10((double)(p.dtlt * (float)p.ndyn));
+  for (int n = 0; n < MECH_NPHOT; ++n) f[MECH_NS + 3 + n] = (float)p.jphoto[n + (long long)MECH_NPHOT * c.o];
+  f[kSurNin - 1] = (float)log10((double)(p.dtlt * (float)p.ndyn));
 #pragma unroll
   for (int k = 0; k < kSurNin; ++k) x[k] = (__bf16)((f[k] - mu[k]) / sd[k]);
 #pragma unroll
@@ -224,10 +228,10 @@ struct SurApplyParams {
   const float* out[4];        // member outputs [64][nrows] (normalised increments)
   long long nrows;
   int nmembers;
-  const float* tmu;           // [47] target mean / std
+  const float* tmu;           // [MECH_NS] target mean / std
   const float* tsd;
-  const float* natoms;        // [47] nitrogen atoms per molecule
-  const float* catoms;        // [47] carbon atoms per molecule
+  const float* natoms;        // [MECH_NS] nitrogen atoms per molecule
+  const float* catoms;        // [MECH_NS] carbon atoms per molecule
   double floor_c;             // concentration floor (molec/cm^3) of the log features
   double sig_c;               // species below this (before and after) skip the spread check
   float spread_tol;           // max relative ensemble spread (e.g. 0.02)

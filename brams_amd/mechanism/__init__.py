@@ -1,8 +1,13 @@
-"""RELACS_TUV mechanism tables (species, reactions, stoichiometry).
+"""Chemical mechanism tables (species, reactions, stoichiometry).
 
-The tables live in relacs_tuv.json, extracted once from the reference's
-generated SPACK kernels by extract_relacs_tuv.py; they are data with the
-reference's species order (chem1_list.f90:63-109) and reaction numbering.
+The tables live in <mechanism>.json (relacs_tuv, relacs_mx, cb07_tuv,
+racm_tuv), extracted once from the reference's generated SPACK kernels by
+extract_relacs_tuv.py; they are data with the reference's species order
+(e.g. RELACS_TUV/chem1_list.f90:63-109) and reaction numbering.
+
+A process works with one mechanism, as a BRAMS build does: the environment
+variable BRAMS_MECH names it (relacs_tuv by default), and the library, the
+generated kernels and the oracle of that mechanism are the ones loaded.
 """
 import json
 import os
@@ -12,7 +17,19 @@ from typing import List
 
 import numpy as np
 
-JSON_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "relacs_tuv.json")
+HERE = os.path.dirname(os.path.abspath(__file__))
+MECHANISMS = ("relacs_tuv", "relacs_mx", "cb07_tuv", "racm_tuv")
+
+
+def default_name():
+    """The process's mechanism: BRAMS_MECH, relacs_tuv by default."""
+    name = os.environ.get("BRAMS_MECH", "relacs_tuv").lower()
+    if name not in MECHANISMS:
+        raise ValueError("BRAMS_MECH=%s is not one of %s" % (name, ", ".join(MECHANISMS)))
+    return name
+
+
+JSON_PATH = os.path.join(HERE, "relacs_tuv.json")
 
 
 @dataclass
@@ -79,8 +96,13 @@ class Mechanism:
         return out
 
 
+def load_mechanism(name=None):
+    """The tables of mechanism `name` (the process's mechanism by default)."""
+    return _load(os.path.join(HERE, (name or default_name()) + ".json"))
+
+
 @lru_cache(maxsize=None)
-def load_mechanism(path=JSON_PATH):
+def _load(path):
     with open(path) as fh:
         d = json.load(fh)
     species = [Species(s["name"], float(np.float32(float(s["weight"]))), bool(s["transported"]))

@@ -21,7 +21,9 @@ the stoichiometry.  This script checks that the derivation reproduces the
 reference's Jacobian term for term (same entries, same term order, same
 coefficients), so the derived code sums in the reference's order.
 
-Usage:  python extract_relacs_tuv.py [/root/reference]
+Usage:  python extract_relacs_tuv.py [/root/reference] [MECHANISM_DIR]
+        (MECHANISM_DIR under src/brams/ccatt: RELACS_TUV by default, or
+        RACM_TUV, CB07_TUV, RELACS_MX; the output is <mechanism_dir>.json)
 """
 import json
 import os
@@ -71,8 +73,7 @@ def parse_species(root):
             idx = int(m.group(2))
             if idx == len(names) + 1:
                 names.append(m.group(1))
-    assert len(names) == 47, len(names)
-    weights, alloc, nr, nphot = None, None, None, None
+    weights, alloc, nr, nphot, ns = None, None, None, None, None
     for s in stmts:
         if s.startswith("REAL,PARAMETER,DIMENSION(nspecies) :: weight="):
             body = s.split("(/", 1)[1].rsplit("/)", 1)[0]
@@ -83,10 +84,15 @@ def parse_species(root):
         m = re.match(r"INTEGER,PARAMETER :: nr\s*=\s*(\d+)$", s)
         if m:
             nr = int(m.group(1))
+        m = re.match(r"INTEGER,PARAMETER :: nspecies\s*=\s*0*(\d+)$", s)
+        if m:
+            ns = int(m.group(1))
         m = re.search(r"nr_photo=\s*(\d+)", s)
         if m:
             nphot = int(m.group(1))
-    assert len(weights) == 47 and len(alloc) == 6 * 47
+    names = names[:ns]
+    assert len(names) == ns, (len(names), ns)
+    assert len(weights) == ns and len(alloc) == 6 * ns, (len(weights), len(alloc), ns)
     species = []
     for i, n in enumerate(names):
         species.append(OrderedDict(name=n, weight=weights[i].rstrip("."),
@@ -141,12 +147,12 @@ def parse_jacobian(root):
     stmts = read_statements(os.path.join(root, MECH_DIR, "chem_spack_jacdchemdc.f90"))
     jac = OrderedDict()
     for s in stmts:
-        m = re.match(r"JacC\(ijkbeg:ijkend,\s*(\d+),\s*(\d+)\)\s*=(.*)$", s)
+        m = re.match(r"JacC\((?:ijkbeg:ijkend|ijk),\s*(\d+),\s*(\d+)\)\s*=(.*)$", s)
         if m:
             key = (int(m.group(1)), int(m.group(2)))
             assert key not in jac, key          # every entry assigned once
             jac[key] = [(c, r, sp) for c, _, r, sp in
-                        parse_terms(m.group(3), r"dw\(ijkbeg:ijkend,\s*(\d+),\s*(\d+)\)")]
+                        parse_terms(m.group(3), r"dw\((?:ijkbeg:ijkend|ijk),\s*(\d+),\s*(\d+)\)")]
     return jac
 
 
@@ -175,9 +181,10 @@ def shape_of(expr):
     A '-' or '+' directly before a literal is folded into the literal when it is
     a unary sign (start of expression or after '(' / operator)."""
     e = expr
-    e = re.sub(r"\(\s*ijkbeg\s*:\s*ijkend\s*\)", "", e, flags=re.I)
-    e = re.sub(r"Jphoto\(\s*ijkbeg\s*:\s*ijkend\s*,\s*(\d+)\)", r"J[\1]", e, flags=re.I)
-    e = re.sub(r"rk\(\s*ijkbeg\s*:\s*ijkend\s*,\s*(\d+)\)", r"RK[\1]", e, flags=re.I)
+    e = re.sub(r"\(\s*(?:ijkbeg\s*:\s*ijkend|ijk)\s*\)", "", e, flags=re.I)
+    e = re.sub(r"Jphoto\(\s*(?:ijkbeg\s*:\s*ijkend|ijk)\s*,\s*(\d+)\)", r"J[\1]", e, flags=re.I)
+    e = re.sub(r"rk\(\s*(?:ijkbeg\s*:\s*ijkend|ijk)\s*,\s*(\d+)\)", r"RK[\1]", e, flags=re.I)
+    e = re.sub(r"\bxlw\b", "vapp", e, flags=re.I)
     e = e.replace(" ", "")
     e = re.sub(r"\btemp\b", "T", e, flags=re.I)
     e = re.sub(r"\bSumM\b", "M", e, flags=re.I)
@@ -217,11 +224,11 @@ def parse_kinetic(root, nr, nphot):
     laws = {}
     tmp = {}
     for s in stmts:
-        m = re.match(r"(Effko|Rapk|facteur)\(ijkbeg:ijkend\)\s*=(.*)$", s, flags=re.I)
+        m = re.match(r"(Effko|Rapk|facteur)\((?:ijkbeg:ijkend|ijk)\)\s*=(.*)$", s, flags=re.I)
         if m:
             tmp[m.group(1).lower()] = shape_of(m.group(2))
             continue
-        m = re.match(r"rk\(ijkbeg:ijkend,\s*(\d+)\)\s*=(.*)$", s, flags=re.I)
+        m = re.match(r"rk\((?:ijkbeg:ijkend|ijk),\s*(\d+)\)\s*=(.*)$", s, flags=re.I)
         if not m:
             continue
         r = int(m.group(1))
@@ -230,6 +237,8 @@ def parse_kinetic(root, nr, nphot):
             law = laws[r]
             if sh == "RK[%d]*M*#" % r:
                 law["post"].append(["mul_M_const", nums[0]])
+            elif sh == "RK[%d]*M" % r:
+                law["post"].append(["mul_M"])
             elif sh == "RK[%d]*W" % r:
                 law["post"].append(["mul_H2O"])
             elif sh == "facteur*RK[%d]" % r:
@@ -268,6 +277,22 @@ def parse_kinetic(root, nr, nphot):
             law["family"], law["params"] = "hno3_oh", e_n + r_n + f_n
         elif sh.startswith("#*EXP("):
             law["family"], law["params"] = "exp_sum", parse_exp_sum(sh, nums)
+        elif sh == "#*M*EXP(#/T)*(#+#*W*EXP(#/T))":
+            c1, b1, one, c2, b2 = nums
+            assert one == 1.0
+            law["family"], law["params"] = "m_exp_h2o", [c1, b1, c2, b2]  # c1 M e^(b1/T) (1 + c2 H2O e^(b2/T))
+        elif sh == "(Effko/(#+Effko/(Rapk/M)))*#**(#+(DLOG10(Effko/(Rapk/M))**#)**(#))":
+            one1, fc, one2, two, mone = nums
+            assert (one1, one2, two, mone) == (1.0, 1.0, 2.0, -1.0)
+            (e_sh, e_n), (r_sh, r_n) = tmp["effko"], tmp["rapk"]
+            assert e_sh == "#*(T/#)**(#)" and r_sh == "#*(T/#)**(#)", (e_sh, r_sh)
+            assert e_n[1] == 300.0 and r_n[1] == 300.0
+            # Effko = k0 (T/300)^n, Rapk = kinf (T/300)^m,
+            # k = Effko/(1 + Effko/(Rapk/M)) Fc^(1 + (log10(Effko/(Rapk/M))^2)^-1)
+            # (the reference's parentheses put the -1 power on the square only)
+            law["family"], law["params"] = "troe_ca", [e_n[0], e_n[2], r_n[0], r_n[2], fc]
+        elif sh == "#*W*W":
+            law["family"], law["params"] = "h2o_sq", nums                # c H2O H2O
         elif sh == "#*(#+#*M)":
             c, one, d = nums
             assert one == 1.0
@@ -353,8 +378,27 @@ def derive_jacobian(reactants, prod, nspec):
     return jac
 
 
+def parse_prod_loss(root):
+    """fexprod (prod = + c w_r) and fexloss (loss = + c dw_rs), term lists."""
+    out = []
+    for fname, lhs, ref in (("chem_spack_fexprod.f90", "prod", r"w\(ijk,\s*(\d+)\)"),
+                            ("chem_spack_fexloss.f90", "loss", r"dw\(ijk,\s*(\d+),\s*(\d+)\)")):
+        rows = {}
+        for s in read_statements(os.path.join(root, MECH_DIR, fname)):
+            m = re.match(lhs + r"\(ijk,\s*(\d+)\)\s*=(.*)$", s)
+            if m:
+                sp = int(m.group(1))
+                assert sp not in rows
+                rows[sp] = parse_terms(m.group(2), ref)
+        out.append(rows)
+    return out
+
+
 def main():
+    global MECH_DIR
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    mech = sys.argv[2] if len(sys.argv) > 2 else "RELACS_TUV"
+    MECH_DIR = "src/brams/ccatt/" + mech
     species, nr, nphot = parse_species(root)
     reactants = parse_reactants(root, nr)
     prod = parse_production(root)
@@ -377,9 +421,20 @@ def main():
     assert set(jac) == set(jac_ref), set(jac) ^ set(jac_ref)
     for key, terms in jac_ref.items():
         assert [(c, r, s) for c, r, s in terms] == jac[key], (key, terms, jac[key])
+    # QSSA's split: fexprod = the positive net terms c w_r, fexloss = -c dw_rs
+    # over the negative ones, both in reaction order
+    p_ref, l_ref = parse_prod_loss(root)
+    for sp in range(1, len(species) + 1):
+        terms = prod.get(sp, [])
+        assert [(c, r) for c, r in terms if c > 0] == [(c, r) for c, _, r in p_ref.get(sp, [])], sp
+        assert [(-c, r) for c, r in terms if c < 0] == [(c, r) for c, _, r, s2 in l_ref.get(sp, [])], sp
+        assert all(s2 == sp for _, _, _, s2 in l_ref.get(sp, []))
 
     data = OrderedDict()
-    data["name"] = "RELACS_TUV"
+    data["name"] = mech
+    cm = [re.search(r"chemical_mechanism\s*=\s*'(\w+)'", x) for x in
+          read_statements(os.path.join(root, MECH_DIR, "chem1_list.f90"))]
+    data["chemical_mechanism"] = [c.group(1) for c in cm if c][0]
     data["note"] = ("Indices are 1-based as in the reference. Rate-law families are "
                     "documented in brams_amd/mechanism/README.md.")
     data["nspecies"] = len(species)
@@ -388,7 +443,7 @@ def main():
     data["species"] = species
     data["reactions"] = [OrderedDict(reactants=reactants[r], rate=laws[r]) for r in range(nr)]
     data["production"] = [[sp, [[c, r] for c, r in prod[sp]]] for sp in sorted(prod)]
-    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "relacs_tuv.json")
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), mech.lower() + ".json")
     with open(out, "w") as fh:
         json.dump(data, fh, indent=1)
     nj = len(jac)

@@ -97,18 +97,26 @@ def alloc_index(grid, block_size=64):
 
 
 def jphoto_from_sza(grid, sza_col):
-    """jphoto (17,m1,m2,m3) float64 for per-column solar zenith angles (deg):
-    CLEAR_SKY_J * cos(SZA), zero beyond 90 deg, with the RELACS relations for
-    slots 14 and 16; levels 1 and m1 copy their neighbours."""
+    """jphoto (nr_photo,m1,m2,m3) float64 for per-column solar zenith angles
+    (deg): CLEAR_SKY_J * cos(SZA), zero beyond 90 deg, with the RELACS
+    relations for slots 14 and 16 (other mechanisms: RELACS_TUV's magnitudes
+    repeated over their rates); levels 1 and m1 copy their neighbours."""
     m1, m2, m3 = grid.shape
     nk, ni, nj = grid.nz, grid.nx, grid.ny
     sza_col = np.asarray(sza_col, dtype=np.float64)
     cosz = np.where(sza_col < 90.0, np.cos(np.deg2rad(sza_col)), 0.0)
-    J = CLEAR_SKY_J[None, None, None, :] * cosz[None, :, :, None]     # (1, ni, nj, 17)
-    J = np.broadcast_to(J, (nk, ni, nj, 17)).copy()
-    J[..., 13] = 0.962055 * J[..., 12] + 0.0106247 * J[..., 8]
-    J[..., 15] = 12.0 * J[..., 12] + 208.0 * J[..., 14]
-    jph = np.zeros((17, m1, m2, m3), dtype=np.float64, order="F")
+    mech = load_mechanism()
+    nph = mech.nr_photo
+    if mech.name == "RELACS_TUV":
+        peak = CLEAR_SKY_J
+    else:                               # synthetic magnitudes for other mechanisms' rates
+        peak = CLEAR_SKY_J[np.arange(nph) % len(CLEAR_SKY_J)]
+    J = peak[None, None, None, :] * cosz[None, :, :, None]              # (1, ni, nj, nph)
+    J = np.broadcast_to(J, (nk, ni, nj, nph)).copy()
+    if mech.name == "RELACS_TUV":
+        J[..., 13] = 0.962055 * J[..., 12] + 0.0106247 * J[..., 8]
+        J[..., 15] = 12.0 * J[..., 12] + 208.0 * J[..., 14]
+    jph = np.zeros((nph, m1, m2, m3), dtype=np.float64, order="F")
     jph[:, 1:m1 - 1, grid.ia - 1:grid.iz, grid.ja - 1:grid.jz] = np.moveaxis(J, 3, 0)
     jph[:, 0] = jph[:, 1]
     jph[:, m1 - 1] = jph[:, m1 - 2]

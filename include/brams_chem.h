@@ -254,6 +254,45 @@ int brams_chem_rodas3_dyndt_device_grids(const brams_chem_dev_args *grids, int32
 int brams_chem_alloc_index(int32_t block_size, int32_t m1, int32_t ia, int32_t iz,
                            int32_t ja, int32_t jz, int32_t *block_end, int32_t max_nob);
 
+/*
+ * The mechanism the library is built for and its sizes (no reference
+ * counterpart: a BRAMS build compiles one mechanism's chem1_list.f90, e.g.
+ * src/brams/ccatt/RELACS_TUV/chem1_list.f90:11-14).  libbrams_chem.so holds
+ * RELACS_TUV (the BRAMS_CHEM_N* values above); libbrams_chem_<mech>.so holds
+ * RELACS_MX, CB07_TUV or RACM_TUV, whose brams_chem_args.nspecies, nr and
+ * nr_photo must equal what this returns.  Any pointer may be NULL.
+ */
+const char *brams_chem_mechanism(int32_t *nspecies, int32_t *nr, int32_t *nr_photo);
+
+/*
+ * dt history checkpoint.  AllocIndex starts every block's step-size history
+ * at dtlt*N_DYN_CHEM when a run starts (chem_spack_utils.f90:170-172), and its
+ * comment there plans to keep that history in the history (restart) file so
+ * that a restarted run reproduces the uninterrupted one; the reference never
+ * writes it.  These calls do: pack writes one grid's last_accepted_dt with
+ * the grid, block size and mechanism it belongs to and a checksum into a
+ * byte record that the caller stores with its history file; unpack checks a
+ * record against the grid it is restored into and writes last_accepted_dt
+ * back.  A step's results depend on the species and this array only, so the
+ * two restore a run exactly.  Host memory only (a device caller copies the
+ * nob doubles across); no device work.  No reference counterpart.
+ */
+typedef struct brams_dt_history_grid {
+  int32_t m1, m2, m3, ia, iz, ja, jz;  /* extents, 1-based interior bounds */
+  int32_t block_size;                  /* AllocIndex's block size (64 here) */
+} brams_dt_history_grid;
+/* Bytes of the record of a grid of nob blocks (80 + 8*nob). */
+int64_t brams_chem_dt_history_bytes(int32_t nob);
+/* Writes the record into buf[0..cap); returns its size.  -8: nob is not the
+ * grid's block count; -7: a dt is not finite and positive. */
+int64_t brams_chem_dt_history_pack(const brams_dt_history_grid *grid, const double *last_accepted_dt,
+                                   int32_t nob, void *buf, int64_t cap);
+/* Checks buf[0..nbytes) and writes last_accepted_dt[0..nob); returns nob.
+ * -7: not a record, another version, corrupt (checksum) or a bad dt; -8: a
+ * record of another grid, block size or mechanism. */
+int brams_chem_dt_history_unpack(const void *buf, int64_t nbytes, const brams_dt_history_grid *grid,
+                                 double *last_accepted_dt, int32_t max_nob);
+
 /* Counters of the last step. */
 int brams_chem_get_stats(brams_chem_stats *out);
 

@@ -14,7 +14,10 @@
  */
 #define _GNU_SOURCE
 #include "brams_oracle.h"
-#include "relacs_tuv_tables.h"
+#ifndef ORA_TABLES
+#define ORA_TABLES "relacs_tuv_tables.h"
+#endif
+#include ORA_TABLES
 #include <math.h>
 #include <pthread.h>
 #include <stdatomic.h>
@@ -97,11 +100,26 @@ void ora_kinetic(double temp, double press, double vapp, const double *jphoto, d
         case 10: /* c (1 + d M), kinetic.f90:229 */
             k = p[0] * (1.0 + p[1] * summ);
             break;
+        case 11: /* c1 M e^(b1/T) (1 + c2 H2O e^(b2/T)), CB07_TUV/chem_spack_kinetic.f90:119-121 */
+            k = p[0] * summ * exp(p[1] / temp) * (1.0 + p[2] * h2o * exp(p[3] / temp));
+            break;
+        case 12: /* c H2O H2O, CB07_TUV/chem_spack_kinetic.f90:220 */
+            k = p[0] * h2o * h2o;
+            break;
+        case 13: { /* chemical-activation fall-off, CB07_TUV/chem_spack_kinetic.f90:231-234 */
+            double effko = p[0] * pow(temp / 300.0, p[1]);
+            double rapk = p[2] * pow(temp / 300.0, p[3]);
+            double x = log10(effko / (rapk / summ));
+            /* Fc^(1 + (x^2)^-1): the reference's parentheses, not 1/(1 + x^2) */
+            k = (effko / (1.0 + effko / (rapk / summ))) * pow(p[4], 1.0 + pow(pow(x, 2.0), -1.0));
+            break;
+        }
         }
         switch (ora_post[r]) {
         case 1: k = k * summ * ora_post_params[r][0]; break;          /* e.g. kinetic.f90:100 */
         case 2: k = k * h2o; break;                                   /* kinetic.f90:109 */
         case 3: k = ora_troe(temp, summ, ora_post_params[r]) * k; break; /* kinetic.f90:178 */
+        case 4: k = k * summ; break;                                  /* CB07_TUV kinetic.f90:273 */
         default: break;
         }
         rk[r] = k;
@@ -1266,8 +1284,14 @@ void ora_prod_loss(const double *rk, const double *y, double *prod, double *loss
                 double term = c * w[r];
                 p = np++ ? p + term : term;
             } else {
-                double dw = rk[r];
-                if (ora_react[r][1] >= 0) dw = dw * y[ora_react[r][0] == s ? ora_react[r][1] : ora_react[r][0]];
+                /* dw_rs exists only where s is a reactant of r; a negative
+                 * product coefficient (CB07_TUV) gives a fexloss term on a dw
+                 * that dratedc never assigns, 0 here */
+                double dw = 0.0;
+                if (ora_react[r][0] == s || ora_react[r][1] == s) {
+                    dw = rk[r];
+                    if (ora_react[r][1] >= 0) dw = dw * y[ora_react[r][0] == s ? ora_react[r][1] : ora_react[r][0]];
+                }
                 double term = (-c) * dw;
                 l = nl++ ? l + term : term;
             }
@@ -1411,8 +1435,8 @@ int ora_qssa_gather_cell(const brams_chem_args *a, long cell, double *y, double 
  * in the reference's loop order: for k = 1..kj with xref(k) /= 0, over the
  * interior columns, jphoto(xref(k), m+1, i, j) = valJLoc(i, j, k, m) for
  * m = 1..m1-1 and jphoto(xref(k), 1, i, j) = jphoto(xref(k), 2, i, j); then
- * RELACS's relations over the whole array (:1363-1368), the constants as
- * REAL literals.  valj is valJLoc(ia:iz, ja:jz, kj, kz), Fortran order;
+ * the mechanism's relations over the whole array (:1333-1368; RELACS: J14
+ * and J16), the constants as REAL literals.  valj is valJLoc(ia:iz, ja:jz, kj, kz), Fortran order;
  * jphoto (np, m1, m2, m3).  Returns -1 on invalid arguments. */
 int ora_tuv_jphoto(int m1, int m2, int m3, int ia, int iz, int ja, int jz, int kj, int kz,
                    const double *valj, const int32_t *xref, int np, double *jphoto)
@@ -1431,14 +1455,22 @@ int ora_tuv_jphoto(int m1, int m2, int m3, int ia, int iz, int ja, int jz, int k
                 JP(n, 0, i, j) = JP(n, 1, i, j);
             }
     }
-    for (int j = 0; j < m3; j++)
-        for (int i = 0; i < m2; i++)
-            for (int k = 0; k < m1; k++)
-                JP(13, k, i, j) = (double)0.962055f * JP(12, k, i, j) + (double)0.0106247f * JP(8, k, i, j);
-    for (int j = 0; j < m3; j++)
-        for (int i = 0; i < m2; i++)
-            for (int k = 0; k < m1; k++)
-                JP(15, k, i, j) = (12.0 * JP(12, k, i, j)) + (208.0 * JP(14, k, i, j));
+    /* the relations of the build's mechanism, each a whole-array statement
+     * in the reference's order */
+#define ALL for (int j = 0; j < m3; j++) for (int i = 0; i < m2; i++) for (int k = 0; k < m1; k++)
+#if ORA_MECH_ID == 1   /* RELACS, :1363-1368 */
+    ALL JP(13, k, i, j) = (double)0.962055f * JP(12, k, i, j) + (double)0.0106247f * JP(8, k, i, j);
+    ALL JP(15, k, i, j) = (12.0 * JP(12, k, i, j)) + (208.0 * JP(14, k, i, j));
+#elif ORA_MECH_ID == 2 /* RACM, :1349-1361 */
+    ALL JP(14, k, i, j) = (double)0.28f * JP(8, k, i, j);
+    ALL JP(19, k, i, j) = (double)0.2f * JP(0, k, i, j);
+    ALL JP(13, k, i, j) = JP(12, k, i, j);
+    ALL JP(15, k, i, j) = (JP(15, k, i, j) + (double)8.7e-07f) / 2.0;
+    ALL JP(22, k, i, j) = JP(15, k, i, j);
+#elif ORA_MECH_ID == 3 /* CB07, :1335-1339 */
+    ALL JP(14, k, i, j) = JP(13, k, i, j);
+#endif
+#undef ALL
 #undef JP
     return 0;
 }

@@ -1,6 +1,7 @@
-// CPU check of the generated RELACS_TUV device functions, compiled for the
-// host: the accessor-templated kernels of relacs_tuv_slab.h against the
-// plain ones of relacs_tuv_gen.h and against the oracle, bit for bit.
+// CPU check of the generated device functions of one mechanism (relacs_tuv
+// by default; -DBRAMS_MECH_GEN/-DBRAMS_MECH_SLAB name another), compiled for
+// the host: the accessor-templated kernels of <mech>_slab.h against the
+// plain ones of <mech>_gen.h and against that mechanism's oracle, bit for bit.
 //
 //   mech_kinetic_t == mech_kinetic == ora_kinetic
 //   mech_fun_t     == mech_fun     == ora_fexchem
@@ -16,7 +17,8 @@
 #include <cstring>
 #include <random>
 
-#include "generated/relacs_tuv_slab.h"
+#include "mechanism.h"
+#include BRAMS_MECH_SLAB
 
 extern "C" {
 void ora_kinetic(double temp, double press, double vapp, const double *jphoto, double *rk);

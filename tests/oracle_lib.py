@@ -9,11 +9,16 @@ import os
 
 import numpy as np
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
-REF_SO = os.path.join(ROOT, "oracle", "_ref", "libbrams_ref.so")
+from brams_amd.mechanism import default_name, load_mechanism
 
-NS, NR, NP = 47, 128, 17
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MECH = default_name()
+_SUFFIX = "" if MECH == "relacs_tuv" else "_" + MECH
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle%s.so" % _SUFFIX)
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libbrams_ref%s.so" % _SUFFIX)
+
+_M = load_mechanism()
+NS, NR, NP = _M.nspecies, _M.nr, _M.nr_photo
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
