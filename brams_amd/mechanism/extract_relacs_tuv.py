@@ -435,8 +435,9 @@ def main():
     cm = [re.search(r"chemical_mechanism\s*=\s*'(\w+)'", x) for x in
           read_statements(os.path.join(root, MECH_DIR, "chem1_list.f90"))]
     data["chemical_mechanism"] = [c.group(1) for c in cm if c][0]
-    data["note"] = ("Indices are 1-based as in the reference. Rate-law families are "
-                    "documented in brams_amd/mechanism/README.md.")
+    data["note"] = ("Indices are 1-based as in the reference. The rate-law families are the device "
+                    "functions rl_<family> of brams_amd/csrc/mech_families.h; extract_relacs_tuv.py "
+                    "documents the kinetic shapes it recognises.")
     data["nspecies"] = len(species)
     data["nr"] = nr
     data["nr_photo"] = nphot

@@ -13,8 +13,13 @@ does; tests/test_mechanisms.py and tests/test_gpu_mechanisms.py start it).
   --gpu  the library built for the mechanism (libbrams_chem_<mech>.so)
          against the oracle: the RODAS3 step (both tolerance settings), the
          ROS2 step (from a relaxed state, see gpu()) and the QSSA step on
-         16x16x10, and the TUV mapping with the mechanism's J relations bit
-         for bit.
+         16x16x10, the TUV mapping with the mechanism's J relations bit for
+         bit, and several grids in one pass (grid by grid in a lean library)
+         against one call per grid, bit for bit.
+  --gpu-scale  the RODAS3 step at full depth against the oracle on sampled
+         blocks (tests/test_gpu_scale_parity.py's check): 50x50x40 at dt
+         30 s, whose first 32 blocks have 65 cells (128-lane workgroups), and
+         the 1.6M-cell 200x200x40 grid.
 
 Prints one line per check and exits non-zero on the first failure.
 """
@@ -163,6 +168,28 @@ def gpu():
     got = dj.cpu().numpy().reshape(before.shape, order="F")
     assert np.array_equal(got, O.tuv_jphoto(grid, valj, xref, before)), "tuv mapping"
     say("GPU TUV mapping with the J relations: bit-identical")
+    from test_gpu_forms_grids import test_grids_in_one_pass_match_one_call_per_grid
+    test_grids_in_one_pass_match_one_call_per_grid()
+    say("GPU several grids in one pass = one call per grid, bit for bit (two calls)")
+
+
+def gpu_scale():
+    # last_accepted_dt at the general bar (rel 1e-6): FMA contraction in the
+    # error norm moves RACM_TUV's controller output by about 1e-9
+    from test_gpu_scale_parity import run_case
+    st = run_case(Grid(50, 50, 40, dtlt=30.0), "split", "baseline", every=25, head=40, dt_rtol=1e-6)
+    say("GPU RODAS3 50x50x40 (dt 30 s, 65-cell blocks on 128 lanes): sampled blocks = oracle, %.1f ms"
+        % st.kernel_ms)
+    # without the x50 NOx/VOC plume: on CB07_TUV and RACM_TUV it blows up
+    # hundreds of blocks (forced errors past 1e30), and which of them end that
+    # way is decided by round-off.  CB07_TUV's cold step out of the seeded
+    # state does so even without it (about 200 blocks blown up, 1400 more
+    # forced past 1e6); its check takes the second step, where the blocks
+    # that blew up are the only ones forced that far.
+    relax = MECH.name == "CB07_TUV"
+    st = run_case(Grid(200, 200, 40), "split", "reference", every=125, dt_rtol=1e-6, plume=False, relax=relax)
+    say("GPU RODAS3 200x200x40 (1.6M cells, no plume%s): sampled blocks = oracle, %.1f ms"
+        % (", second step" if relax else "", st.kernel_ms))
 
 
 if __name__ == "__main__":
@@ -173,4 +200,6 @@ if __name__ == "__main__":
         cpu()
     if "--gpu" in sys.argv:
         gpu()
+    if "--gpu-scale" in sys.argv:
+        gpu_scale()
     say("all checks passed")
