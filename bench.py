@@ -122,6 +122,11 @@ BYTES_PER_CELL_ATTEMPT = {
     "rodas3_blocks<64,false,true,1>": 29300 + 8500,
     "ros2_blocks<64,4>": 31300 + 9350,          # profiles/ros2_kernel_stats_and_traffic.md
 }
+# the same for the lean library of another mechanism, by (mechanism, kernel):
+# profiles/other_mechanisms_rodas3.md
+BYTES_PER_CELL_ATTEMPT_MECH = {
+    ("RACM_TUV", "rodas3_blocks<64,false,false,1>"): 78700 + 19000,
+}
 MLP_BYTES_PER_CELL = 1825
 FORM_WPS = {3: 1, 4: 2, 5: 4}
 
@@ -373,8 +378,10 @@ def bench_plain(args, solver="ros2"):
                    "flop_model_" + solver: {"per_cell_call": f_call, "per_substep": f_sub, "per_attempt": f_att}},
         "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
-                     # spill traffic measured for RELACS_TUV's kernels only
-                     "traffic": None if lean else spill_traffic(kernel, att_main / K),
+                     # spill traffic per cell-attempt, where it was measured
+                     "traffic": (BYTES_PER_CELL_ATTEMPT_MECH.get((mech.name, kernel)) * att_main / K
+                                 if (mech.name, kernel) in BYTES_PER_CELL_ATTEMPT_MECH
+                                 else None if lean else spill_traffic(kernel, att_main / K)),
                      "kernel": kernel, "avg_ms": main_ms / K,
                      "algorithmic_bytes": (2 * mech.nspecies * 4 + 4 * 4 + mech.nr_photo * 8) * grid.ncell},
     }
