@@ -9,7 +9,8 @@ does; tests/test_mechanisms.py and tests/test_gpu_mechanisms.py start it).
          Sparse1.3 (its own Markowitz ordering) against the oracle's fixed
          pivot order: equal attempts, sub-steps and mask, outputs within
          rel 1e-6, and no pivot of the fixed order failing Sparse's
-         threshold rule on the matrix Sparse orders on (cell 1 of block 1).
+         threshold rule on the matrix Sparse orders on (cell 1 of block 1);
+         the ROS2 step likewise, from a relaxed state (see gpu()).
   --gpu  the library built for the mechanism (libbrams_chem_<mech>.so)
          against the oracle: the RODAS3 step (both tolerance settings), the
          ROS2 step (from a relaxed state, see gpu()) and the QSSA step on
@@ -87,6 +88,39 @@ def cpu():
             say("RODAS3 step %s/%s: oracle = reference Sparse1.3 path (attempts %d, mask %d, worst rel %.2g, "
                 "threshold failures on the ordering matrix 0, elsewhere %d)"
                 % (sza, tol, st.attempts, int(mask.sum()), rel, st.threshold_pivots))
+    # ROS2 from the relaxed state of gpu(): the oracle's fixed order against
+    # the reference's Sparse1.3 ordered on its first factorisation of the
+    # full pattern (chem_spack_solve_sparse.f90); equal attempts per block,
+    # outputs within rel 1e-6 or 4x the one-ulp sensitivity
+    wr, sens = relaxed_state(Grid(16, 16, 10))
+    atol, rtol = tolerances("reference")
+    nob = len(wr["block_end"])
+    hs, st, mask, rc = O.run_oracle(wr, atol, rtol, solver="ros2")
+    a1, s1 = O.block_counts(nob)
+    hr, sr, maskr, rcr = O.run_oracle(wr, atol, rtol, solver="ros2", reference_sparse=True)
+    a2, s2 = O.block_counts(nob)
+    assert rc == 0 and rcr == 0 and np.array_equal(a1, a2) and np.array_equal(s1, s2)
+    rep = parity_report(hr.sc_p, hs.sc_p, Grid(16, 16, 10), wr, rel=max(1e-6, 4 * sens))
+    assert rep["nfail"] == 0, (rep["worst"], sens)
+    say("ROS2 step (relaxed state): oracle = reference Sparse1.3 path (attempts %d per block equal, worst %.2g, "
+        "one-ulp sensitivity %.2g)" % (st.attempts, rep["worst"], sens))
+
+
+def relaxed_state(grid):
+    """The seeded split-SZA state after one RODAS3 step of the oracle at the
+    reference tolerances, and the oracle ROS2 step's worst change under a
+    one-ulp perturbation of it."""
+    wl = make_workload(grid, seed=2, sza="split")
+    atol, rtol = tolerances("reference")
+    h0, _, _, _ = O.run_oracle(wl, atol, rtol)
+    wr = dict(wl)
+    wr["sc_p"] = [np.asfortranarray(np.asarray(x, np.float32)) for x in h0.sc_p]
+    wr["last_accepted_dt"] = np.asarray(h0.last_dt).copy()
+    hs, _, _, _ = O.run_oracle(wr, atol, rtol, solver="ros2")
+    wp = dict(wr)
+    wp["sc_p"] = [np.asfortranarray(np.nextafter(x, np.float32(np.inf))) for x in wr["sc_p"]]
+    hp, _, _, _ = O.run_oracle(wp, atol, rtol, solver="ros2")
+    return wr, parity_report(hp.sc_p, hs.sc_p, grid, wr)["worst"]
 
 
 def gpu():
@@ -122,22 +156,14 @@ def gpu():
     # values are then defined by round-off only); from the relaxed state no
     # block is forced.  The bar: rel 1e-6, or 4x the oracle's own change under
     # a one-ulp perturbation of the same input if that is larger.
-    wl = make_workload(grid, seed=2, sza="split")
+    wr, sens = relaxed_state(grid)
     atol, rtol = tolerances("reference")
-    h0, _, _, _ = O.run_oracle(wl, atol, rtol)
-    wr = dict(wl)
-    wr["sc_p"] = [np.asfortranarray(np.asarray(x, np.float32)) for x in h0.sc_p]
-    wr["last_accepted_dt"] = np.asarray(h0.last_dt).copy()
     ds = DeviceStep(wr, atol, rtol, solver="ros2")
     rc = ds.step()
     st = abi.stats()
     gpu_sc = ds.species_host()
     del ds
     hs, ost, mask, orc = O.run_oracle(wr, atol, rtol, solver="ros2")
-    wp = dict(wr)
-    wp["sc_p"] = [np.asfortranarray(np.nextafter(x, np.float32(np.inf))) for x in wr["sc_p"]]
-    hp, _, _, _ = O.run_oracle(wp, atol, rtol, solver="ros2")
-    sens = parity_report(hp.sc_p, hs.sc_p, grid, wr)["worst"]
     rep = parity_report(gpu_sc, hs.sc_p, grid, wr, rel=max(1e-6, 4 * sens))
     assert rc == 0 and orc == 0 and st.attempts == ost.attempts and ost.forced_blocks == 0 and rep["nfail"] == 0, \
         (rc, orc, st.attempts, ost.attempts, ost.forced_blocks, rep["worst"], sens)
