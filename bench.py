@@ -123,7 +123,7 @@ BYTES_PER_CELL_ATTEMPT = {
     "ros2_blocks<64,4>": 31300 + 9350,          # profiles/ros2_kernel_stats_and_traffic.md
 }
 # the same for the lean library of another mechanism, by (mechanism, kernel):
-# profiles/other_mechanisms_rodas3.md
+# profiles/other_mechanisms_solvers.md
 BYTES_PER_CELL_ATTEMPT_MECH = {
     ("RACM_TUV", "rodas3_blocks<64,false,false,1>"): 78700 + 19000,
 }
