@@ -78,8 +78,10 @@ struct DevBuf {
 constexpr int kFormPerLane = brams::kSlabVariants;      // first per-lane form
 constexpr int kPerLaneWps[3] = {1, 2, 4};
 constexpr int kForms = brams::kSlabVariants + 3;
-// per-lane forms the library holds (a lean build: 1 wave per SIMD only)
-constexpr int kPerLaneBuilt = brams::kLeanBuild ? 1 : 3;
+// the per-lane forms the library holds (a lean build: 1 and 4 waves per SIMD)
+constexpr bool form_built(int f) {
+  return f >= kFormPerLane && f < kForms && (!brams::kLeanBuild || f != kFormPerLane + 1);
+}
 // List (fallback) launches: the per-lane kernel at 1 or 2 waves per SIMD,
 // by the launch's block count.  Measured on the 1.6M-cell rollout (ms per
 // launch, 1 / 2 waves per SIMD): 19 400 blocks 246 / 190; the 2000..6000
@@ -382,7 +384,8 @@ int autotune(brams::StepParams p, int first64, int nob, hipStream_t st) {
   q.last_dt = dt1;
   const int f0 = kSlabEnabled ? 0 : kFormPerLane;
   int best = f0;
-  for (int f = f0; f < kFormPerLane + kPerLaneBuilt; ++f) {
+  for (int f = f0; f < kForms; ++f) {
+    if (f >= kFormPerLane && !form_built(f)) continue;
     HIP_TRY(hipMemcpyAsync(sc1, sc0, nsc, hipMemcpyDeviceToDevice, st));
     if (to_t) HIP_TRY(hipMemcpyAsync(st1, st0, nsc, hipMemcpyDeviceToDevice, st));
     HIP_TRY(hipMemcpyAsync(dt1, dt0, sizeof(double) * nob, hipMemcpyDeviceToDevice, st));
@@ -1599,7 +1602,7 @@ int brams_chem_set_form(int32_t form) {
   if (form < -1 || form >= kForms) return fail(-2, "kernel form %d out of range (-1..%d)", form, kForms - 1);
   if (!kSlabEnabled && form >= 0 && form < kFormPerLane)
     return fail(-6, "kernel form %d (slab) is switched off in this build", form);
-  if (form >= kFormPerLane + kPerLaneBuilt)
+  if (form >= kFormPerLane && !form_built(form))
     return fail(-6, "kernel form %d is not built into this %s library (lean build)", form, MECH_NAME);
   g.force_form = form;
   return 0;

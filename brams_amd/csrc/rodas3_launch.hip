@@ -22,14 +22,11 @@ hipError_t launch_rodas3(int wg, bool dyn, unsigned nblocks, const StepParams& p
     hipLaunchKernelGGL((rodas3_blocks<64, true, false, 1>), dim3(nblocks), dim3(64), 0, st, p, block0);
     return hipGetLastError();
   }
-  if (wps == 0) wps = (!kLeanBuild && nblocks >= (unsigned)kRodas3ManyBlocks) ? kRodas3ManyWaves : 1;
+  if (wps == 0) wps = nblocks >= (unsigned)kRodas3ManyBlocks ? kRodas3ManyWaves : 1;
   if (wps == 2) return launch_rodas3_64w2(nblocks, p, block0, st);
-#ifndef BRAMS_MECH_LEAN
   if (wps == 4)
     hipLaunchKernelGGL((rodas3_blocks<64, false, false, 4>), dim3(nblocks), dim3(64), 0, st, p, block0);
-  else
-#endif
-  if (wps == 1)
+  else if (wps == 1)
     hipLaunchKernelGGL((rodas3_blocks<64, false, false, 1>), dim3(nblocks), dim3(64), 0, st, p, block0);
   else
     return hipErrorInvalidValue;
