@@ -11,11 +11,12 @@ struct StepParams;
 // Kernel forms built into the library.  A library built with
 // BRAMS_MECH_LEAN (the mechanisms other than RELACS_TUV) instantiates the
 // per-lane RODAS3 and ROS2 kernels at one wave per SIMD for 64 and 128
-// lanes, and the 64-lane RODAS3 kernel also at four (the autotuner keeps the
-// faster); neither the two-wave, list (surrogate fallback), several-grid
-// (rodas3_grids; the entry launches grid by grid instead) nor slab kernels:
-// each instantiation of a large mechanism takes minutes to compile.  The
-// launchers below return hipErrorInvalidValue for a form not built.
+// lanes, and the 64-lane ones also at four (RODAS3: the autotuner keeps the
+// faster; ROS2: from kRodas3ManyBlocks blocks on); neither the two-wave, list
+// (surrogate fallback), several-grid (rodas3_grids; the entry launches grid
+// by grid instead) nor slab kernels: each instantiation of a large mechanism
+// takes minutes to compile.  The launchers below return hipErrorInvalidValue
+// for a form not built.
 #ifdef BRAMS_MECH_LEAN
 constexpr bool kLeanBuild = true;
 #else

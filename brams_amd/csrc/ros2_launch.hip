@@ -11,10 +11,8 @@ hipError_t launch_ros2(int wg, unsigned nblocks, const StepParams& p, int block0
   if (nblocks == 0) return hipSuccess;
   if (wg == 128)
     hipLaunchKernelGGL((ros2_blocks<128, 1>), dim3(nblocks), dim3(128), 0, st, p, block0);
-#ifndef BRAMS_MECH_LEAN
   else if (wg == 64 && nblocks >= (unsigned)kRodas3ManyBlocks)
     hipLaunchKernelGGL((ros2_blocks<64, kRodas3ManyWaves>), dim3(nblocks), dim3(64), 0, st, p, block0);
-#endif
   else if (wg == 64)
     hipLaunchKernelGGL((ros2_blocks<64, 1>), dim3(nblocks), dim3(64), 0, st, p, block0);
   else

@@ -20,7 +20,7 @@ does; tests/test_mechanisms.py and tests/test_gpu_mechanisms.py start it).
   --gpu-scale  the RODAS3 step at full depth against the oracle on sampled
          blocks (tests/test_gpu_scale_parity.py's check): 50x50x40 at dt
          30 s, whose first 32 blocks have 65 cells (128-lane workgroups), and
-         the 1.6M-cell 200x200x40 grid.
+         the 1.6M-cell 200x200x40 grid; and the ROS2 step on that grid.
 
 Prints one line per check and exits non-zero on the first failure.
 """
@@ -216,7 +216,55 @@ def gpu_scale():
     st = run_case(Grid(200, 200, 40), "split", "reference", every=125, dt_rtol=1e-6, plume=False, relax=relax)
     say("GPU RODAS3 200x200x40 (1.6M cells, no plume%s): sampled blocks = oracle, %.1f ms"
         % (", second step" if relax else "", st.kernel_ms))
+    gpu_scale_ros2()
 
+
+def gpu_scale_ros2():
+    """ROS2 on the 1.6M-cell grid (the four-wave kernel from 4096 blocks on)
+    against the oracle on block 1 and 200 sampled blocks, from the state one
+    RODAS3 step of the GPU leaves (reference tolerances; out of the seeded
+    state ROS2 accepts at dt_min nearly everywhere and a one-ulp change of
+    the input moves its output by up to 1e27).  Equal attempts and sub-steps
+    per block, the same blocks failed (those a blown-up RODAS3 step left
+    non-finite), and outputs within rel 1e-6 or four times the oracle's own
+    change under a one-ulp perturbation of the input, whichever is larger."""
+    from brams_amd import abi
+    from brams_amd.chem import DeviceStep
+    from test_gpu_scale_parity import compare, sample_blocks
+    grid = Grid(200, 200, 40)
+    print("case ros2 200x200x40 split reference, relaxed state, 201 sampled blocks", flush=True)
+    wl = make_workload(grid, seed=0, sza="split")
+    atol, rtol = tolerances("reference")
+    nob = len(wl["block_end"])
+    ds = DeviceStep(wl, atol, rtol)
+    ds.step()
+    wr = dict(wl)
+    wr["sc_p"] = ds.species_host()
+    wr["last_accepted_dt"] = ds.last_dt.cpu().numpy().copy()
+    del ds
+    ds = DeviceStep(wr, atol, rtol, solver="ros2")
+    rc = ds.step()
+    st = abi.stats()
+    gpu = ds.species_host()
+    gatt, gsub, gfail, _ = abi.block_status(nob)
+    del ds
+    blocks = np.union1d(sample_blocks(nob, 125), np.nonzero(gfail)[0][np.nonzero(gfail)[0] > 0]).astype(np.int32)
+    ran = np.concatenate([[0], blocks])
+    hs, ost, _, orc = O.run_oracle_blocks(wr, atol, rtol, blocks, solver="ros2")
+    oatt, osub = O.block_counts(nob)
+    ofail = O.block_failed(nob)
+    wp = dict(wr)
+    wp["sc_p"] = [np.asfortranarray(np.nextafter(np.asarray(x, np.float32), np.float32(np.inf))) for x in wr["sc_p"]]
+    hp, _, _, _ = O.run_oracle_blocks(wp, atol, rtol, blocks, solver="ros2")
+    ids = ran[(gfail[ran] == 0) & (ofail[ran] == 0)]
+    sens, _ = compare(grid, wr, hp.sc_p, hs.sc_p, ids)
+    worst, _ = compare(grid, wr, gpu, hs.sc_p, ids)
+    assert np.array_equal(gfail[ran] != 0, ofail[ran] != 0) and (rc > 0) == (orc > 0), (rc, orc)
+    assert np.array_equal(gatt[ids], oatt[ids]) and np.array_equal(gsub[ids], osub[ids])
+    assert worst <= max(1e-6, 4 * sens), (worst, sens)
+    say("GPU ROS2 200x200x40 (four waves per SIMD, relaxed state): attempts per sampled block = oracle, "
+        "%d failed blocks on both sides, worst %.2g (oracle's one-ulp sensitivity %.2g), %.1f ms"
+        % (int(np.count_nonzero(gfail)), worst, sens, st.kernel_ms))
 
 if __name__ == "__main__":
     if MECH.name == "RELACS_TUV":
