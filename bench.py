@@ -356,10 +356,9 @@ def bench_plain(args, solver="ros2"):
         att_main += st.cell_attempts * share
         log("%s step %d: %.1f ms (main kernel %.1f ms, %d cell-attempts)"
             % (solver, t, st.kernel_ms, st.main_ms, st.cell_attempts))
-    n64 = sum(1 for b in wl["block_end"] if b <= 64)
     lean = mech.name != "RELACS_TUV"        # a lean library (traffic measured where listed above)
     if solver == "ros2":
-        kernel = "ros2_blocks<64,%d>" % (4 if n64 >= 4096 else 1)      # ros2_launch.hip's shape
+        kernel = "ros2_blocks<64,%d>" % FORM_WPS.get(abi.stats().kernel_form, 0)   # the autotuned form
     else:
         kernel = "rodas3_blocks<64,false,false,%d>" % FORM_WPS.get(abi.tuning()[0], 0)
     achieved = flops_main / (main_ms * 1e-3) / 1e12

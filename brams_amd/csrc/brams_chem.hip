@@ -113,6 +113,8 @@ struct GridState {
   std::vector<float> h_forced;
   int form = -1;                 // kernel form of the 64-cell blocks (-1: not tuned yet)
   float tune_ms[kForms] = {};
+  int ros2_form = -1;            // the same for the ROS2 step (forms 3 and 5)
+  float ros2_tune_ms[kForms] = {};
   bool same(int a1, int a2, int a3, int b1, int b2, int b3, int b4) const {
     return m1 == a1 && m2 == a2 && m3 == a3 && ia == b1 && iz == b2 && ja == b3 && jz == b4;
   }
@@ -469,6 +471,39 @@ constexpr StructMask kStructMask = struct_mask();
 // grid: every block alike on the full structural Jacobian, so no block-1 form
 // and no mask; the 65..127-cell blocks on 128 lanes, the 64-cell blocks
 // longest first from the second call on the grid.
+// The ROS2 step's 64-cell blocks at one and four waves per SIMD (forms 3
+// and 5, both built in every library), timed on a scratch copy of the state
+// on the first ROS2 call of a grid, as autotune() does for RODAS3; the
+// blocks are independent (no mask), so no other block runs first.
+int ros2_autotune(const brams::StepParams& p, int first64, int nob, hipStream_t st) {
+  GridState& gs = *g.cur;
+  const size_t nsc = sizeof(float) * MECH_NS * (size_t)p.n3d;
+  const bool to_t = (p.split == brams::kSplitParallel) && (p.ndyn == 1);
+  HIP_TRY(g.tune_sc.ensure(nsc * (to_t ? 2 : 1)));
+  HIP_TRY(g.tune_dt.ensure(sizeof(double) * nob));
+  brams::StepParams q = p;
+  q.sc_p = g.tune_sc.as<float>();
+  q.sc_t = to_t ? q.sc_p + (size_t)MECH_NS * p.n3d : nullptr;
+  q.last_dt = g.tune_dt.as<double>();
+  const int forms[2] = {kFormPerLane, kFormPerLane + 2};
+  int best = forms[0];
+  for (int f : forms) {
+    HIP_TRY(hipMemcpyAsync(q.sc_p, p.sc_p, nsc, hipMemcpyDeviceToDevice, st));
+    if (to_t) HIP_TRY(hipMemcpyAsync(q.sc_t, p.sc_t, nsc, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(q.last_dt, p.last_dt, sizeof(double) * nob, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipEventRecord(g.ev0, st));
+    HIP_TRY(brams::launch_ros2(64, (unsigned)(nob - first64), q, first64, st, kPerLaneWps[f - kFormPerLane]));
+    HIP_TRY(hipEventRecord(g.ev1, st));
+    HIP_TRY(hipEventSynchronize(g.ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+    gs.ros2_tune_ms[f] = ms;
+    if (ms < gs.ros2_tune_ms[best]) best = f;
+  }
+  gs.ros2_form = best;
+  return 0;
+}
+
 int launch_ros2_step(brams::StepParams p, hipStream_t st) {
   GridState& gs = *g.cur;
   const int nob = (int)gs.sizes.size();
@@ -484,6 +519,15 @@ int launch_ros2_step(brams::StepParams p, hipStream_t st) {
   HIP_TRY(hipMemsetAsync(gs.order_thresh.p, 0, sizeof(int), st));
   int nbig = 0;
   while (nbig < nob && gs.sizes[nbig] > 64) ++nbig;
+  // brams_chem_set_form forces the ROS2 form too (3 or 5: ROS2 has no
+  // two-wave kernel)
+  if (g.force_form == kFormPerLane + 1)
+    return fail(-6, "ROS2 has no kernel form %d (two waves per SIMD)", g.force_form);
+  if (g.force_form < 0 && gs.ros2_form < 0 && nob > nbig) {
+    int rc = ros2_autotune(p, nbig, nob, st);
+    if (rc) return rc;
+  }
+  const int form = g.force_form >= 0 ? g.force_form : gs.ros2_form;
   const int* order = nullptr;
   if (nob > nbig && gs.blk_hist) {
     int rc = order_blocks(nbig, nob - nbig, st);
@@ -498,12 +542,12 @@ int launch_ros2_step(brams::StepParams p, hipStream_t st) {
   if (nob > nbig) {
     brams::StepParams pm = p;
     pm.block_order = order;
-    HIP_TRY(brams::launch_ros2(64, (unsigned)(nob - nbig), pm, nbig, st));
+    HIP_TRY(brams::launch_ros2(64, (unsigned)(nob - nbig), pm, nbig, st, kPerLaneWps[form - kFormPerLane]));
     for (int b = nbig; b < nob; ++b) g.main_cells += gs.sizes[b];
   }
   HIP_TRY(hipEventRecord(g.ev1, st));
   gs.blk_hist = true;
-  g.last_form = -1;
+  g.last_form = nob > nbig ? form : -1;
   return 0;
 }
 

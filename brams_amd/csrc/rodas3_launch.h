@@ -37,9 +37,9 @@ hipError_t launch_rodas3_128(bool dyn, unsigned nblocks, const StepParams& p, in
 hipError_t launch_rodas3_64w2(unsigned nblocks, const StepParams& p, int block0, hipStream_t st);
 
 // The ROS2 solver (ros2_launch.hip): nblocks workgroups of wg (64 or 128)
-// lanes over blocks block0.. of p; the 64-lane form takes four waves per SIMD
-// from kRodas3ManyBlocks blocks on, else one.
-hipError_t launch_ros2(int wg, unsigned nblocks, const StepParams& p, int block0, hipStream_t st);
+// lanes over blocks block0.. of p; wps: waves per SIMD of the 64-lane form
+// (1 or 4; 0: four from kRodas3ManyBlocks blocks on, else one).
+hipError_t launch_ros2(int wg, unsigned nblocks, const StepParams& p, int block0, hipStream_t st, int wps = 0);
 
 // The QSSA solver (qssa_launch.hip): one lane per interior cell, itchim
 // sub-steps; nfatal (device int, zeroed by the caller) counts the cells where

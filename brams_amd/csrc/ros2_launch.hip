@@ -7,13 +7,14 @@
 
 namespace brams {
 
-hipError_t launch_ros2(int wg, unsigned nblocks, const StepParams& p, int block0, hipStream_t st) {
+hipError_t launch_ros2(int wg, unsigned nblocks, const StepParams& p, int block0, hipStream_t st, int wps) {
   if (nblocks == 0) return hipSuccess;
+  if (wg == 64 && wps == 0) wps = nblocks >= (unsigned)kRodas3ManyBlocks ? kRodas3ManyWaves : 1;
   if (wg == 128)
     hipLaunchKernelGGL((ros2_blocks<128, 1>), dim3(nblocks), dim3(128), 0, st, p, block0);
-  else if (wg == 64 && nblocks >= (unsigned)kRodas3ManyBlocks)
+  else if (wg == 64 && wps == kRodas3ManyWaves)
     hipLaunchKernelGGL((ros2_blocks<64, kRodas3ManyWaves>), dim3(nblocks), dim3(64), 0, st, p, block0);
-  else if (wg == 64)
+  else if (wg == 64 && wps == 1)
     hipLaunchKernelGGL((ros2_blocks<64, 1>), dim3(nblocks), dim3(64), 0, st, p, block0);
   else
     return hipErrorInvalidValue;

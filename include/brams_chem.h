@@ -315,7 +315,10 @@ int brams_chem_block_thresholds(int32_t *counts, int32_t max_nob);
  * per-slot scratch slab, persistent workgroups over a device work queue) are
  * switched off in this build and refused (-6).  -1 (default): the first call
  * of each grid times every available form on a scratch copy of its state and
- * keeps the fastest for the grid.
+ * keeps the fastest for the grid.  The ROS2 step has forms 3 and 5 and tunes
+ * its own choice the same way; with form 4 forced its calls fail (-6).  A
+ * lean library (libbrams_chem_<mech>.so) has forms 3 and 5 only and refuses
+ * form 4 here (-6).
  */
 int brams_chem_set_form(int32_t form);
 /* The current grid's chosen form and the autotuner's timings (ms, one per

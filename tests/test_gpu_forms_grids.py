@@ -50,6 +50,38 @@ def test_register_budget_forms_are_bit_identical_and_tuned():
     assert torch.equal(ds.sc_p, outs[0][0]) and torch.equal(ds.last_dt, outs[0][1])
 
 
+def test_ros2_forms_are_bit_identical_and_tuned():
+    """The ROS2 step at one and four waves per SIMD (forms 3 and 5) gives
+    bit-identical results; form 4 (two waves) does not exist for ROS2 and is
+    refused; the first ROS2 call of a grid times both and keeps one."""
+    import torch
+    from brams_amd.chem import DeviceStep
+    print("case ros2 forms 3/5, 24x24x10 seed 2", flush=True)
+    grid = Grid(24, 24, 10)
+    wl = make_workload(grid, seed=2, sza="split")
+    atol, rtol = tolerances("baseline")
+    outs = []
+    try:
+        for form in (3, 5):
+            abi.set_form(form)
+            ds = DeviceStep(wl, atol, rtol, solver="ros2")
+            assert ds.step() == 0 and abi.stats().kernel_form == form
+            outs.append((ds.sc_p.clone(), ds.last_dt.clone()))
+            del ds
+        abi.set_form(4)
+        ds = DeviceStep(wl, atol, rtol, solver="ros2")
+        with pytest.raises(abi.ChemError):
+            ds.step()
+        del ds
+    finally:
+        abi.set_form(-1)
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+    abi.lib().brams_chem_finalize()                     # forget the grid: the next call tunes
+    ds = DeviceStep(wl, atol, rtol, solver="ros2")
+    assert ds.step() == 0 and abi.stats().kernel_form in (3, 5)
+    assert torch.equal(ds.sc_p, outs[0][0]) and torch.equal(ds.last_dt, outs[0][1])
+
+
 NESTED = [(Grid(40, 40, 10, dtlt=120.0), 0, "split"),
           (Grid(98, 98, 4, dtlt=60.0), 1, "split_night_first"),
           (Grid(50, 50, 6, dtlt=30.0), 2, "random")]
