@@ -37,6 +37,21 @@ def test_library_exports(name):
     assert hasattr(lib, name), name
 
 
+@pytest.mark.parametrize("mech", ["relacs_mx", "cb07_tuv", "racm_tuv"])
+def test_mechanism_libraries_export_the_same_entry_points(mech):
+    """libbrams_chem_<mech>.so (the lean build of another mechanism) has the
+    same C-ABI, and brams_chem_mechanism names its mechanism."""
+    path = abi.lib_path(mech)
+    if not os.path.exists(path):
+        pytest.skip("%s not built" % os.path.basename(path))
+    lib = C.CDLL(path)
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    lib.brams_chem_mechanism.restype = C.c_char_p
+    lib.brams_chem_mechanism.argtypes = [C.c_void_p] * 3
+    assert lib.brams_chem_mechanism(None, None, None).decode() == mech.upper()
+
+
 @pytest.mark.parametrize("nx,ny,nz", [(16, 16, 10), (98, 98, 40), (50, 50, 40), (3, 3, 12), (1, 1, 5)])
 def test_alloc_index_through_c_abi(nx, ny, nz):
     from brams_amd.chem import alloc_index as c_alloc
