@@ -15,8 +15,9 @@ does; tests/test_mechanisms.py and tests/test_gpu_mechanisms.py start it).
          against the oracle: the RODAS3 step (both tolerance settings), the
          ROS2 step (from a relaxed state, see gpu()) and the QSSA step on
          16x16x10, the TUV mapping with the mechanism's J relations bit for
-         bit, and several grids in one pass (grid by grid in a lean library)
-         against one call per grid, bit for bit.
+         bit, several grids in one pass (grid by grid in a lean library)
+         against one call per grid, bit for bit, and the split write-backs
+         through the host entry.
   --gpu-scale  the RODAS3 step at full depth against the oracle on sampled
          blocks (tests/test_gpu_scale_parity.py's check): 50x50x40 at dt
          30 s, whose first 32 blocks have 65 cells (128-lane workgroups), and
@@ -197,6 +198,55 @@ def gpu():
     from test_gpu_forms_grids import test_grids_in_one_pass_match_one_call_per_grid
     test_grids_in_one_pass_match_one_call_per_grid()
     say("GPU several grids in one pass = one call per grid, bit for bit (two calls)")
+    for method, ndyn in (("SEQUENTIAL", 1), ("PARALLEL", 1), ("PARALLEL", 2)):
+        worst, sens = split_case(method, ndyn)
+        say("GPU split write-back %s N_DYN_CHEM=%d through the host entry: worst %.2g (oracle's one-ulp "
+            "sensitivity %.2g)" % (method, ndyn, worst, sens))
+
+
+def split_case(method, ndyn):
+    """The write-back forms of chem_spack_rodas3_dyndt.f90:713-798 through the
+    host entry on the 16x16x6 workload of tests/test_gpu_rodas3.py (SEQUENTIAL
+    overwrites sc_p; PARALLEL with N_DYN_CHEM=1 adds (new - old)/dtlt to sc_t;
+    N_DYN_CHEM>1 starts from sc_p - sc_t_dyn*N*dtlt).  Every block of it
+    accepts at dt_min on these mechanisms, so the bar on the written array is
+    rel 1e-6 (+1e-12 of its largest value), or four times the oracle's own
+    change under a one-ulp perturbation of the input, whichever is larger."""
+    from brams_amd.chem import run_workload_host
+    grid = Grid(16, 16, 6)
+    wl = make_workload(grid, seed=4, sza="split", plume_sigma=3.0)
+    atol, rtol = tolerances("reference")
+    code = {"SEQUENTIAL": 1, "PARALLEL": 2}[method]
+    sc_t = sc_t_dyn = None
+    if method == "PARALLEL" and ndyn == 1:
+        rng = np.random.default_rng(4)
+        sc_t = [np.asfortranarray((rng.standard_normal(a.shape) * 1e-6).astype(np.float32)) for a in wl["sc_p"]]
+    if method == "PARALLEL" and ndyn > 1:
+        sc_t_dyn = [np.asfortranarray((-0.01 * a / (ndyn * grid.dtlt)).astype(np.float32)) for a in wl["sc_p"]]
+    gpu, dt_gpu, rc, gpu_t = run_workload_host(wl, atol, rtol, split_method=method, n_dyn_chem=ndyn,
+                                               sc_t=sc_t, sc_t_dyn=sc_t_dyn)
+    hs, _, _, orc = O.run_oracle(wl, atol, rtol, split_method=code, n_dyn_chem=ndyn, sc_t=sc_t, sc_t_dyn=sc_t_dyn)
+    wp = dict(wl)
+    wp["sc_p"] = [np.asfortranarray(np.nextafter(np.asarray(x, np.float32), np.float32(np.inf))) for x in wl["sc_p"]]
+    hp, _, _, _ = O.run_oracle(wp, atol, rtol, split_method=code, n_dyn_chem=ndyn, sc_t=sc_t, sc_t_dyn=sc_t_dyn)
+    assert rc == 0 and orc == 0, (rc, orc)
+    np.testing.assert_allclose(dt_gpu, hs.last_dt, rtol=1e-6)
+    t1 = method == "PARALLEL" and ndyn == 1
+    if t1:
+        for s, (a, b) in enumerate(zip(gpu, wl["sc_p"])):         # sc_p untouched
+            assert np.array_equal(a, b), s
+    got, ref, pert = (gpu_t, hs.sc_t, hp.sc_t) if t1 else (gpu, hs.sc_p, hp.sc_p)
+
+    def rel(x, y):
+        x = np.asarray(x, np.float64)
+        y = np.asarray(y, np.float64)
+        # rel <= 1e-6 is |x - y| <= 1e-6 |y| + 1e-12 max|y|
+        return np.abs(x - y) / (np.abs(y) + 1e-6 * max(np.abs(y).max(), 1e-30))
+
+    sens = max(float(rel(c, r).max()) for c, r in zip(pert, ref))
+    worst = max(float(rel(g, r).max()) for g, r in zip(got, ref))
+    assert worst <= max(1e-6, 4 * sens), (method, ndyn, worst, sens)
+    return worst, sens
 
 
 def gpu_scale():
