@@ -125,7 +125,12 @@ BYTES_PER_CELL_ATTEMPT = {
 # the same for the lean library of another mechanism, by (mechanism, kernel):
 # profiles/other_mechanisms_solvers.md
 BYTES_PER_CELL_ATTEMPT_MECH = {
+    ("RELACS_MX", "rodas3_blocks<64,false,false,1>"): 29900 + 8400,
+    ("RELACS_MX", "rodas3_blocks<64,false,false,4>"): 49100 + 14100,
+    ("CB07_TUV", "rodas3_blocks<64,false,false,1>"): 17300 + 5000,
+    ("CB07_TUV", "rodas3_blocks<64,false,false,4>"): 30800 + 9700,
     ("RACM_TUV", "rodas3_blocks<64,false,false,1>"): 78700 + 19000,
+    ("RACM_TUV", "rodas3_blocks<64,false,false,4>"): 111700 + 31300,
 }
 MLP_BYTES_PER_CELL = 1825
 FORM_WPS = {3: 1, 4: 2, 5: 4}
