@@ -2,8 +2,11 @@
  * brams_chem.h -- C-ABI of the MI355X CCATT gas-phase chemistry step.
  *
  * Drop-in boundary for BRAMS's RODAS3 chemistry solver (CHEMISTRY=4,
- * mechanism RELACS_TUV).  The Fortran module mod_chem_spack_rodas3_dyndt in
- * brams_amd/fortran/ keeps the reference interface of
+ * mechanism RELACS_TUV; libbrams_chem_<mech>.so gives the same entry points
+ * for RELACS_MX, CB07_TUV and RACM_TUV, see brams_chem_mechanism), with the
+ * ROS2 (CHEMISTRY=3) and QSSA (CHEMISTRY=1) solvers beside it.  The Fortran
+ * module mod_chem_spack_rodas3_dyndt in brams_amd/fortran/ keeps the
+ * reference interface of
  *   chem_rodas3_dyndt   src/brams/ccatt/chem_spack_rodas3_dyndt.f90:41-131
  * and calls brams_chem_rodas3_dyndt() below through ISO_C_BINDING.
  *
