@@ -21,7 +21,8 @@ does; tests/test_mechanisms.py and tests/test_gpu_mechanisms.py start it).
   --gpu-scale  the RODAS3 step at full depth against the oracle on sampled
          blocks (tests/test_gpu_scale_parity.py's check): 50x50x40 at dt
          30 s, whose first 32 blocks have 65 cells (128-lane workgroups), and
-         the 1.6M-cell 200x200x40 grid; and the ROS2 step on that grid.
+         the 1.6M-cell 200x200x40 grid; and the ROS2 and QSSA steps on that
+         grid.
 
 Prints one line per check and exits non-zero on the first failure.
 """
@@ -267,6 +268,11 @@ def gpu_scale():
     say("GPU RODAS3 200x200x40 (1.6M cells, no plume%s): sampled blocks = oracle, %.1f ms"
         % (", second step" if relax else "", st.kernel_ms))
     gpu_scale_ros2()
+    # QSSA on every cell of the 1.6M-cell grid with the plume, against the
+    # oracle's whole-grid QSSA (tests/test_gpu_qssa.py's check and bar)
+    from test_gpu_qssa import test_qssa_200x200x40_matches_oracle
+    test_qssa_200x200x40_matches_oracle()
+    say("GPU QSSA 200x200x40 (every cell, plume) = oracle within the QSSA bar")
 
 
 def gpu_scale_ros2():
