@@ -16,8 +16,9 @@ does; tests/test_mechanisms.py and tests/test_gpu_mechanisms.py start it).
          ROS2 step (from a relaxed state, see gpu()) and the QSSA step on
          16x16x10, the TUV mapping with the mechanism's J relations bit for
          bit, several grids in one pass (grid by grid in a lean library)
-         against one call per grid, bit for bit, and the split write-backs
-         through the host entry.
+         against one call per grid and the no-allocation entry against the
+         device entry, both bit for bit, and the split write-backs through
+         the host entry.
   --gpu-scale  the RODAS3 step at full depth against the oracle on sampled
          blocks (tests/test_gpu_scale_parity.py's check): 50x50x40 at dt
          30 s, whose first 32 blocks have 65 cells (128-lane workgroups), and
@@ -199,6 +200,9 @@ def gpu():
     from test_gpu_forms_grids import test_grids_in_one_pass_match_one_call_per_grid
     test_grids_in_one_pass_match_one_call_per_grid()
     say("GPU several grids in one pass = one call per grid, bit for bit (two calls)")
+    from test_zz_no_alloc import test_noalloc_entry_matches_device_entry
+    test_noalloc_entry_matches_device_entry()
+    say("GPU no-allocation entry = device entry, bit for bit (three grids in a row, shrinking)")
     for method, ndyn in (("SEQUENTIAL", 1), ("PARALLEL", 1), ("PARALLEL", 2)):
         worst, sens = split_case(method, ndyn)
         say("GPU split write-back %s N_DYN_CHEM=%d through the host entry: worst %.2g (oracle's one-ulp "
