@@ -17,8 +17,9 @@ does; tests/test_mechanisms.py and tests/test_gpu_mechanisms.py start it).
          16x16x10, the TUV mapping with the mechanism's J relations bit for
          bit, several grids in one pass (grid by grid in a lean library)
          against one call per grid and the no-allocation entry against the
-         device entry, both bit for bit, and the split write-backs through
-         the host entry.
+         device entry, both bit for bit, the split write-backs through the
+         host entry, and the Fortran drop-ins linked to the mechanism's
+         library against the C-ABI, bit for bit.
   --gpu-scale  the RODAS3 step at full depth against the oracle on sampled
          blocks (tests/test_gpu_scale_parity.py's check): 50x50x40 at dt
          30 s, whose first 32 blocks have 65 cells (128-lane workgroups), and
@@ -207,6 +208,31 @@ def gpu():
         worst, sens = split_case(method, ndyn)
         say("GPU split write-back %s N_DYN_CHEM=%d through the host entry: worst %.2g (oracle's one-ulp "
             "sensitivity %.2g)" % (method, ndyn, worst, sens))
+    fortran_drop_ins()
+
+
+def fortran_drop_ins():
+    """The Fortran drop-ins (chem_rodas3_dyndt, chem_ros_dyndt, chem_qssa)
+    linked to this mechanism's library (brams_amd/fortran/shim_driver_<mech>)
+    against the C-ABI called from Python: bit for bit."""
+    import pathlib
+    import tempfile
+    import test_gpu_fortran_shim as FS
+    from brams_amd.chem import run_workload_host, run_workload_qssa_host
+    atol, rtol = tolerances("baseline")
+    with tempfile.TemporaryDirectory() as td:
+        tp = pathlib.Path(td)
+        wl = make_workload(Grid(10, 8, 9), seed=11)
+        for solver in ("rodas3", "ros2"):
+            fort, fdt = FS.run_driver(tp, wl, atol, rtol, solver)
+            py, pdt, rc, _ = run_workload_host(wl, atol, rtol, solver=solver)
+            assert rc == 0 and all(np.array_equal(a, b) for a, b in zip(fort, py)) and np.array_equal(fdt, pdt), solver
+        wq = make_workload(Grid(10, 8, 9), seed=11, sza="split", all_ppbm=True)
+        fort, _ = FS.run_driver(tp, wq, atol, rtol, "qssa")
+        py, rc = run_workload_qssa_host(wq)
+        assert rc == 0 and all(np.array_equal(a, b) for a, b in zip(fort, py)), "qssa"
+    say("Fortran drop-ins chem_rodas3_dyndt, chem_ros_dyndt and chem_qssa linked to %s = the C-ABI, bit for bit"
+        % os.path.basename(FS.DRIVER))
 
 
 def split_case(method, ndyn):
