@@ -605,7 +605,7 @@ int check_grid(int m1, int m2, int m3, int ia, int iz, int ja, int jz, int photo
   if (ia < 1 || iz > m2 || ia > iz || ja < 1 || jz > m3 || ja > jz)
     return fail(-2, "bad interior bounds ia=%d iz=%d ja=%d jz=%d", ia, iz, ja, jz);
   if (photoj == BRAMS_PHOTOJ_LUT)
-    return fail(-4, "PhotojMethod LUT is not supported (RELACS_TUV uses FAST-TUV)");
+    return fail(-4, "PhotojMethod LUT is not supported (the TUV mechanisms use FAST-TUV)");
   if (photoj < 0 || photoj > 2) return fail(-2, "unknown PhotojMethod %d", photoj);
   if (!(dtlt > 0.f)) return fail(-2, "dtlt must be positive");
   return 0;
@@ -1331,8 +1331,8 @@ int host_step(const brams_chem_args* a, int solver) {
                         a->split_method, a->photoj_method, a->dtlt);
   if (rc) return rc;
   if (a->nspecies != MECH_NS || a->nr != MECH_NR || a->nr_photo != MECH_NPHOT)
-    return fail(-2, "mechanism sizes %d/%d/%d differ from RELACS_TUV %d/%d/%d", a->nspecies, a->nr,
-                a->nr_photo, MECH_NS, MECH_NR, MECH_NPHOT);
+    return fail(-2, "mechanism sizes %d/%d/%d differ from this library's %s %d/%d/%d", a->nspecies, a->nr,
+                a->nr_photo, MECH_NAME, MECH_NS, MECH_NR, MECH_NPHOT);
   if ((rc = ensure_grid(a->m1, a->m2, a->m3, a->ia, a->iz, a->ja, a->jz))) return rc;
   const int nob = (int)g.cur->sizes.size();
   if (a->nob != nob) return fail(-2, "nob %d does not match the AllocIndex partition (%d)", a->nob, nob);
@@ -1459,7 +1459,8 @@ int qssa_step(const brams::StepParams& p, int itchim, long long ncell, hipStream
 
 int check_qssa(int itchim, int nspecies) {
   if (itchim < 1) return fail(-2, "itchim must be >= 1 (chemistry.f90 passes 75)");
-  if (nspecies != MECH_NS) return fail(-2, "nspecies %d differs from RELACS_TUV %d", nspecies, MECH_NS);
+  if (nspecies != MECH_NS)
+    return fail(-2, "nspecies %d differs from this library's %s %d", nspecies, MECH_NAME, MECH_NS);
   return 0;
 }
 
@@ -1485,8 +1486,8 @@ int brams_chem_qssa(const brams_chem_args* a, int32_t itchim) {
   int rc = check_grid(a->m1, a->m2, a->m3, a->ia, a->iz, a->ja, a->jz, a->photoj_method, a->dtlt);
   if (rc || (rc = check_qssa(itchim, a->nspecies))) return rc;
   if (a->nr != MECH_NR || a->nr_photo != MECH_NPHOT)
-    return fail(-2, "mechanism sizes %d/%d differ from RELACS_TUV %d/%d", a->nr, a->nr_photo, MECH_NR,
-                MECH_NPHOT);
+    return fail(-2, "mechanism sizes %d/%d differ from this library's %s %d/%d", a->nr, a->nr_photo,
+                MECH_NAME, MECH_NR, MECH_NPHOT);
   if (!a->pp || !a->pi0 || !a->theta || !a->rv || !a->jphoto || !a->weight || !a->sc_p || !a->sc_t)
     return fail(-2, "null argument (QSSA needs pp, pi0, theta, rv, jphoto, weight, sc_p, sc_t)");
   for (int s = 0; s < MECH_NS; ++s)
@@ -1542,7 +1543,8 @@ int brams_chem_tuv_jphoto_device(const brams_tuv_map_args* a) {
   if (a->m1 < 3 || a->m2 < 1 || a->m3 < 1 || a->ia < 1 || a->iz > a->m2 || a->ia > a->iz || a->ja < 1 ||
       a->jz > a->m3 || a->ja > a->jz)
     return fail(-2, "bad grid %d %d %d / interior %d..%d %d..%d", a->m1, a->m2, a->m3, a->ia, a->iz, a->ja, a->jz);
-  if (a->nr_photo != MECH_NPHOT) return fail(-2, "nr_photo %d differs from RELACS_TUV %d", a->nr_photo, MECH_NPHOT);
+  if (a->nr_photo != MECH_NPHOT)
+    return fail(-2, "nr_photo %d differs from this library's %s %d", a->nr_photo, MECH_NAME, MECH_NPHOT);
   if (a->kj < 1 || a->kz < a->m1 - 1)
     return fail(-2, "valj must hold kj >= 1 reactions and kz >= m1-1 = %d levels (kj %d, kz %d)", a->m1 - 1,
                 a->kj, a->kz);
