@@ -1,9 +1,13 @@
 /*
  * ORACLE -- test infrastructure only (see brams_oracle.h).
  *
- * CPU restatement of src/brams/ccatt/chem_spack_rodas3_dyndt.f90 and the
- * RELACS_TUV kernels, written to follow the reference's operation order so
- * that the kernels agree bit for bit with the compiled reference.  The one
+ * CPU restatement of src/brams/ccatt/chem_spack_rodas3_dyndt.f90 (and of the
+ * ROS2 and QSSA drivers) and of a mechanism's kernels, written to follow the
+ * reference's operation order so that the kernels agree bit for bit with the
+ * compiled reference.  The mechanism is a build-time choice: ORA_TABLES names
+ * its tables (relacs_tuv_tables.h by default; relacs_mx, cb07_tuv and
+ * racm_tuv build liboracle_<mech>.so), each pinned against that mechanism's
+ * own compiled kernels (oracle/_ref/libbrams_ref_<mech>.so).  The one
  * deliberate difference is the LU pivot order: the reference re-derives a
  * Markowitz order from cell 1 of block 1 on every block-1 attempt
  * (rodas3:470-506, spFactor.c:173-300); this oracle uses one fixed
